@@ -1,0 +1,1263 @@
+// api.cpp -- the libEZmock C ABI on the GPU engine.
+//
+// Host orchestration of the eight reference entry points
+// (libEZmock/EZmock/EZmock.h:76-216) and of the extensions in EZmock_ext.h.
+// Argument validation, error codes and their order follow the reference
+// functions cited at each entry point; the hot path is launched on one HIP
+// stream per instance and its fields stay resident in device memory between
+// calls (the calibration loop re-populates one field many times).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+#include <unistd.h>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "rng.h"
+#include "fft.h"
+#include "kernels.h"
+#include "host_math.h"
+#include "EZmock_ext.h"
+
+// ------------------------------------------------------------ instance
+namespace {
+
+struct Conf { int nthread, Ng; double Lbox; };
+struct RngConf { int type; uint64_t seed; };
+struct Cosmo { double growth2, vfac; };
+struct PkTab {
+  int n; bool log; double kmin, kmax;
+  std::vector<double> k, P, P2;
+  long version;
+};
+struct Par { double bao, rho_c, rho_exp, pdf_base, sigma_v; };
+
+struct Timer { std::string name; hipEvent_t a, b; };
+
+struct Mesh {
+  bool have_field = false, psi_ref = false, fixamp = false, iphase = false;
+  real *psi_host[3] = {nullptr, nullptr, nullptr};
+  bool dev_ready = false;
+  hipStream_t st = nullptr;
+  const uint32_t *jtab = nullptr;
+  // fields (device)
+  real *psi[3] = {nullptr, nullptr, nullptr};
+  real *rho = nullptr;          // also the spectrum buffer before the CIC
+  cplx *ny = nullptr;           // kz = Nyquist plane (+ r2c scratch), 2 Ng^2
+  real *rhot = nullptr;
+  int16_t *nt = nullptr;
+  real *mom[3] = {nullptr, nullptr, nullptr};
+  bool mom_valid = false;
+  double *ptab = nullptr;
+  long ptab_version = -1;
+  cplx *twN = nullptr, *twh = nullptr;
+  real *spec = nullptr;
+  // tracer scratch
+  uint32_t *blockcnt = nullptr;
+  uint64_t *blockoff = nullptr, *scan_tmp = nullptr;
+  unsigned long long *hist = nullptr, *counter = nullptr;
+  rkey *keys = nullptr, *keys_tmp = nullptr;
+  size_t keys_cap = 0;
+  uint32_t *sort_gh = nullptr;
+  uint64_t *sort_go = nullptr, *sort_scan = nullptr;
+  uint64_t *d_num = nullptr, *d_cnt = nullptr, *d_res = nullptr;
+  rkey *d_vb = nullptr;
+  double *d_rsd = nullptr;
+  size_t rsd_cap = 0;
+  real *cat[6] = {nullptr};
+  size_t cat_cap = 0, ntracer = 0;
+  // last PDF
+  size_t nmax = 0, ncell = 0, ntot = 0, ncell0 = 0, ntot0 = 0;
+  std::vector<size_t> pdf;
+  unsigned long long hist_h[EZ_NUM_LOGB_BIN] = {0};
+  // options
+  bool keep_spec = false, profile = false;
+  std::vector<Timer> pending;
+  std::map<std::string, std::pair<double, long>> times;
+};
+
+long g_pk_version = 0;
+
+Conf *CONF(EZMOCK *ez) { return (Conf *) ez->conf; }
+RngConf *RNG(EZMOCK *ez) { return (RngConf *) ez->rng; }
+Cosmo *COSMO(EZMOCK *ez) { return (Cosmo *) ez->cosmo; }
+PkTab *PK(EZMOCK *ez) { return (PkTab *) ez->pk; }
+Par *PAR(EZMOCK *ez) { return (Par *) ez->par; }
+Mesh *MESH(EZMOCK *ez) { return (Mesh *) ez->mesh; }
+
+// ------------------------------------------------------- device helpers
+std::mutex g_jtab_mu;
+std::map<int, uint32_t *> g_jtab;
+
+int device_ready(Mesh *m) {
+  if (m->dev_ready) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    fprintf(stderr, "EZmock: no HIP device available; the density field and "
+        "tracer stages run only on the GPU\n");
+    return EZMOCK_ERR_UNKNOWN;
+  }
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::mutex> lk(g_jtab_mu);
+    auto it = g_jtab.find(dev);
+    if (it == g_jtab.end()) {
+      uint32_t *d = nullptr;
+      HIP_OK(hipMalloc(&d, MRG_JUMP_WORDS * sizeof(uint32_t)));
+      HIP_OK(hipMemcpy(d, mrg_host_jump_table(), MRG_JUMP_WORDS * sizeof(uint32_t),
+          hipMemcpyHostToDevice));
+      g_jtab[dev] = d;
+      m->jtab = d;
+    }
+    else m->jtab = it->second;
+  }
+  m->dev_ready = true;
+  return 0;
+}
+
+template <class T> int dalloc(T **p, size_t count) {
+  if (*p) return 0;
+  if (hipMalloc((void **) p, count * sizeof(T) + 64) != hipSuccess) {
+    *p = nullptr;
+    (void) hipGetLastError();
+    fprintf(stderr, "EZmock: device allocation of %zu bytes failed\n",
+        count * sizeof(T));
+    return EZMOCK_ERR_MEMORY;
+  }
+  return 0;
+}
+
+template <class T> void dfree(T **p) {
+  if (*p) (void) hipFree((void *) *p);
+  *p = nullptr;
+}
+
+void free_tracer_scratch(Mesh *m) {
+  dfree(&m->keys); dfree(&m->keys_tmp); m->keys_cap = 0;
+  dfree(&m->sort_gh); dfree(&m->sort_go); dfree(&m->sort_scan);
+}
+
+void free_mesh_device(Mesh *m) {
+  for (int i = 0; i < 3; i++) { dfree(&m->psi[i]); dfree(&m->mom[i]); }
+  dfree(&m->rho); dfree(&m->ny); dfree(&m->rhot); dfree(&m->nt);
+  dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
+  dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
+  dfree(&m->hist); dfree(&m->counter);
+  free_tracer_scratch(m);
+  dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
+  dfree(&m->d_rsd);
+  for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
+  m->cat_cap = 0;
+  for (auto &t : m->pending) { (void) hipEventDestroy(t.a); (void) hipEventDestroy(t.b); }
+  m->pending.clear();
+  if (m->st) (void) hipStreamDestroy(m->st);
+  m->st = nullptr;
+}
+
+// Kernel-class timing with HIP events on the instance stream (option 2).
+struct Scope {
+  Mesh *m; Timer t; bool on;
+  Scope(Mesh *mm, const char *name) : m(mm), on(mm->profile) {
+    if (!on) return;
+    t.name = name;
+    (void) hipEventCreate(&t.a);
+    (void) hipEventCreate(&t.b);
+    (void) hipEventRecord(t.a, m->st);
+  }
+  ~Scope() {
+    if (!on) return;
+    (void) hipEventRecord(t.b, m->st);
+    m->pending.push_back(t);
+  }
+};
+
+int sync_and_collect(Mesh *m) {
+  HIP_OK(hipStreamSynchronize(m->st));
+  for (auto &t : m->pending) {
+    float ms = 0;
+    (void) hipEventElapsedTime(&ms, t.a, t.b);
+    auto &e = m->times[t.name];
+    e.first += ms;
+    e.second += 1;
+    (void) hipEventDestroy(t.a);
+    (void) hipEventDestroy(t.b);
+  }
+  m->pending.clear();
+  return 0;
+}
+
+#define KCHK(expr)                                                          \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      fprintf(stderr, "EZmock: kernel launch failed (%s) at %s:%d\n",       \
+              hipGetErrorString(e_), __FILE__, __LINE__);                   \
+      return EZMOCK_ERR_UNKNOWN;                                            \
+    }                                                                       \
+  } while (0)
+
+int ensure_twiddles(Mesh *m, int Ng) {
+  if (m->twN) return 0;
+  const int Ngh = Ng / 2;
+  int e;
+  if ((e = dalloc(&m->twN, Ng)) || (e = dalloc(&m->twh, Ngh))) return e;
+  std::vector<cplx> a(Ng), b(Ngh);
+  for (int i = 0; i < Ng; i++) {
+    const double t = 2 * EZ_PI * (double) i / Ng;
+    a[i] = cplx{(real) cos(t), (real) sin(t)};
+  }
+  for (int i = 0; i < Ngh; i++) {
+    const double t = 2 * EZ_PI * (double) i / Ngh;
+    b[i] = cplx{(real) cos(t), (real) sin(t)};
+  }
+  HIP_OK(hipMemcpy(m->twN, a.data(), sizeof(cplx) * Ng, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(m->twh, b.data(), sizeof(cplx) * Ngh, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Inverse 3-D c2r of component `comp` of the generated spectrum (rhok in
+// m->rho viewed as complex, Nyquist plane in m->ny) into out.
+int c2r_component(Mesh *m, int Ng, int comp, real *out) {
+  cplx *buf = (cplx *) out;
+  {
+    Scope s(m, "fft_pass_i");
+    KCHK(fft_strided_pass(buf, (const cplx *) m->rho, m->ny, m->twN, Ng, 0,
+        +1, 1, comp, m->st));
+  }
+  {
+    Scope s(m, "fft_pass_j");
+    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, m->st));
+  }
+  {
+    Scope s(m, "fft_rows_c2r");
+    KCHK(fft_row_pass(buf, m->twN, m->twh, Ng, +1, m->st));
+  }
+  return 0;
+}
+
+// r2c of a real field held in buf (in place): packed result, A/B unpacked.
+int r2c_field(Mesh *m, int Ng, real *field) {
+  cplx *buf = (cplx *) field;
+  {
+    Scope s(m, "fft_rows_r2c");
+    KCHK(fft_row_pass(buf, m->twN, m->twh, Ng, -1, m->st));
+  }
+  {
+    Scope s(m, "fft_pass_j");
+    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, -1, 0, 0, m->st));
+  }
+  {
+    Scope s(m, "fft_pass_i");
+    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 0, -1, 0, 0, m->st));
+  }
+  {
+    Scope s(m, "r2c_unpack");
+    KCHK(launch_unpack_r2c(buf, m->ny, Ng, m->st));
+  }
+  return 0;
+}
+
+// pop_tracer.c:94-133
+int eval_pdf(size_t nexp, real pbase, size_t ncells, std::vector<size_t> &num,
+    size_t &ncell, size_t &ntot) {
+  size_t n;
+  double A = 0, sum = 0, power = pbase;
+  for (n = 1; n <= nexp; n++) {
+    sum += power * n;
+    A = nexp / sum;
+    power *= pbase;
+    if (A * power < 0.5) break;
+  }
+  if (n > EZ_MAX_NUM_TRACER_PER_CELL) return EZMOCK_ERR_RHO_PBASE;
+  num.assign(n, 0);
+  size_t nc = 0, nobj = 0;
+  for (size_t i = 0; i < n; i++) {
+    num[i] = (size_t) round(A * pow(pbase, (double) (n - i)));
+    nc += num[i];
+    nobj += num[i] * (n - i);
+  }
+  if (nc > ncells || nobj == 0) return EZMOCK_ERR_PAR_PBASE;
+  ncell = nc;
+  ntot = nobj;
+  return 0;
+}
+
+int ensure_cat(Mesh *m, size_t n) {
+  if (m->cat_cap >= n && m->cat[0]) return 0;
+  for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
+  m->cat_cap = 0;
+  const size_t cap = n + n / 8 + 1024;
+  for (int i = 0; i < 6; i++) {
+    int e = dalloc(&m->cat[i], cap);
+    if (e) return e;
+  }
+  m->cat_cap = cap;
+  return 0;
+}
+
+// The device part of EZmock_populate_tracer; catalogue left in m->cat.
+int populate_device(EZMOCK *ez, const real *params, size_t nexp,
+    bool att_part, size_t *ntracer) {
+  Conf *conf = CONF(ez);
+  Mesh *m = MESH(ez);
+  Cosmo *cosmo = COSMO(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const real rho_c = params[0], rho_exp = params[1], pbase = params[2],
+      sigma_v = params[3];
+  if (!isfinite(rho_c) || rho_c < 0) return EZMOCK_ERR_PAR_RCUT;
+  if (!isfinite(rho_exp) || rho_exp <= 0) return EZMOCK_ERR_PAR_REXP;
+  if (!isfinite(pbase) || pbase <= 0 || pbase >= 1) return EZMOCK_ERR_PAR_PBASE;
+  if (!isfinite(sigma_v) || sigma_v < 0) return EZMOCK_ERR_PAR_SIGV;
+
+  std::vector<size_t> num;
+  size_t ncell, ntot;
+  int e = eval_pdf(nexp, pbase, N, num, ncell, ntot);
+  if (e) return e;
+  const int nmax = (int) num.size();
+  m->ncell0 = ncell;
+  m->ntot0 = ntot;
+
+  if (att_part) {
+    fprintf(stderr, "EZmock: att_part = true (attach tracers to particles) "
+        "is not available on the GPU engine\n");
+    return EZMOCK_ERR_UNKNOWN;
+  }
+  if ((e = device_ready(m))) return e;
+  if (m->psi_ref) {             // the reference reads the caller's arrays
+    for (int i = 0; i < 3; i++)
+      HIP_OK(hipMemcpyAsync(m->psi[i], m->psi_host[i], N * sizeof(real),
+          hipMemcpyHostToDevice, m->st));
+    m->mom_valid = false;
+  }
+  const size_t nblk = (N + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  if ((e = dalloc(&m->rhot, N)) || (e = dalloc(&m->nt, N)) ||
+      (e = dalloc(&m->blockcnt, nblk)) || (e = dalloc(&m->blockoff, nblk)) ||
+      (e = dalloc(&m->scan_tmp, scan_tmp_words(nblk) + 8)) ||
+      (e = dalloc(&m->hist, EZ_NUM_LOGB_BIN)) || (e = dalloc(&m->counter, 2)) ||
+      (e = dalloc(&m->d_num, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
+      (e = dalloc(&m->d_cnt, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
+      (e = dalloc(&m->d_vb, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
+      (e = dalloc(&m->d_res, 4)))
+    return e;
+
+  // ---- bias model, pop_tracer.c:149-348
+  const real thres = (rho_c == 0) ? (real) EZ_REAL_MIN : rho_c;
+  BiasParams bp;
+  bp.N = N;
+  bp.thres = thres;
+  bp.rho_sat = (real) EZ_RHO_SAT;
+  bp.iexp = -1 / rho_exp;                 // evaluated in `real`
+  const real lambda = (real) EZ_GAUSS_WIDTH;
+  bp.lambda = lambda;
+  const real rho_max = 1 + lambda * 5;
+  bp.imax = ilogb(rho_max);
+  mrg_seed_state(RNG(ez)->seed, bp.seed_state);
+  {
+    Scope s(m, "bias_count");
+    KCHK(launch_count_valid(m->rho, thres, N, m->blockcnt, m->st));
+  }
+  {
+    Scope s(m, "scan");
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+  }
+  uint64_t nvalid = 0;
+  const size_t nch = (nblk + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  HIP_OK(hipMemcpyAsync(&nvalid, m->scan_tmp + nch, sizeof(uint64_t),
+      hipMemcpyDeviceToHost, m->st));
+  if ((e = sync_and_collect(m))) return e;
+  if (nvalid < ncell) return EZMOCK_ERR_RHO_LESS;
+  KCHK(launch_zero(m->hist, EZ_NUM_LOGB_BIN * sizeof(unsigned long long), m->st));
+  {
+    Scope s(m, "bias");
+    KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, bp, m->jtab, m->st));
+  }
+  HIP_OK(hipMemcpyAsync(m->hist_h, m->hist, sizeof(m->hist_h),
+      hipMemcpyDeviceToHost, m->st));
+  if ((e = sync_and_collect(m))) return e;
+
+  // ---- PDF mapping, pop_tracer.c:361-514
+  int imax;
+  size_t cnt = 0;
+  for (imax = 0; imax < EZ_NUM_LOGB_BIN; imax++) {
+    cnt += m->hist_h[imax];
+    if (cnt >= ncell) break;
+  }
+  if (imax == EZ_NUM_LOGB_BIN) return EZMOCK_ERR_UNKNOWN;
+  const real rho_thres = (imax == EZ_NUM_LOGB_BIN - 1) ? (real) EZ_REAL_MIN
+      : (real) pow(2, bp.imax - imax);
+  if (m->keys_cap < cnt) {
+    free_tracer_scratch(m);
+    const size_t cap = cnt + cnt / 4 + 4096;
+    const size_t hw = sort_hist_words(cap);
+    if ((e = dalloc(&m->keys, cap)) || (e = dalloc(&m->keys_tmp, cap)) ||
+        (e = dalloc(&m->sort_gh, hw)) || (e = dalloc(&m->sort_go, hw)) ||
+        (e = dalloc(&m->sort_scan, scan_tmp_words(hw) + 8)))
+      return e;
+    m->keys_cap = cap;
+  }
+  KCHK(launch_zero(m->counter, 2 * sizeof(unsigned long long), m->st));
+  {
+    Scope s(m, "compact");
+    KCHK(launch_compact(m->rhot, rho_thres, N, m->keys, m->keys_cap, m->counter, m->st));
+  }
+  {
+    Scope s(m, "sort");
+    KCHK(launch_sort(m->keys, m->keys_tmp, cnt, m->sort_gh, m->sort_go,
+        m->sort_scan, m->st));
+  }
+  std::vector<uint64_t> numu(num.begin(), num.end());
+  HIP_OK(hipMemcpyAsync(m->d_num, numu.data(), sizeof(uint64_t) * nmax,
+      hipMemcpyHostToDevice, m->st));
+  {
+    Scope s(m, "walk");
+    KCHK(launch_walk(m->keys, cnt, m->d_num, nmax, m->d_vb, m->d_cnt, m->d_res,
+        m->st));
+  }
+  uint64_t res[4] = {0, 0, 0, 0}, ncand = 0;
+  std::vector<uint64_t> pcnt(nmax);
+  HIP_OK(hipMemcpyAsync(res, m->d_res, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
+  HIP_OK(hipMemcpyAsync(&ncand, m->counter, sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
+  HIP_OK(hipMemcpyAsync(pcnt.data(), m->d_cnt, sizeof(uint64_t) * nmax,
+      hipMemcpyDeviceToHost, m->st));
+  if ((e = sync_and_collect(m))) return e;
+  if (ncand != cnt || res[2]) return EZMOCK_ERR_UNKNOWN;
+  ncell = res[0];
+  ntot = res[1];
+  m->nmax = nmax;
+  m->ncell = ncell;
+  m->ntot = ntot;
+  m->pdf.assign(pcnt.begin(), pcnt.end());
+
+  // ---- tracers per cell and positions, pop_tracer.c:881-1007
+  {
+    Scope s(m, "assign_nt");
+    KCHK(launch_assign_nt(m->rhot, N, m->d_vb, nmax, m->nt, m->blockcnt, m->st));
+  }
+  {
+    Scope s(m, "scan");
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+  }
+  if ((e = ensure_cat(m, ntot))) return e;
+  const real gs = conf->Lbox / conf->Ng;
+  const real igs = conf->Ng / conf->Lbox;
+  {
+    Scope s(m, "positions");
+    KCHK(launch_positions(m->nt, m->blockoff, Ng, conf->Lbox, gs, bp.seed_state,
+        m->jtab, m->cat[0], m->cat[1], m->cat[2], ntot, m->st));
+  }
+  // ---- velocities, pop_tracer.c:593-753 and :1043-1113
+  if (!m->mom_valid) {
+    for (int i = 0; i < 3; i++)
+      if ((e = dalloc(&m->mom[i], N))) return e;
+    {
+      Scope s(m, "zero_mom");
+      for (int i = 0; i < 3; i++) KCHK(launch_zero(m->mom[i], N * sizeof(real), m->st));
+    }
+    {
+      Scope s(m, "cic_momentum");
+      KCHK(launch_cic(m->psi[0], m->psi[1], m->psi[2], m->mom[0], m->mom[1],
+          m->mom[2], Ng, igs, 1, m->st));
+    }
+    m->mom_valid = !m->psi_ref;
+  }
+  {
+    Scope s(m, "velocities");
+    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->rho, m->mom[0],
+        m->mom[1], m->mom[2], ntot, Ng, igs, cosmo->vfac, (double) sigma_v,
+        bp.seed_state, m->jtab, m->cat[3], m->cat[4], m->cat[5], m->st));
+  }
+  if ((e = sync_and_collect(m))) return e;
+  m->ntracer = ntot;
+  *ntracer = ntot;
+  Par *par = PAR(ez);
+  par->rho_c = rho_c;
+  par->rho_exp = rho_exp;
+  par->pdf_base = pbase;
+  par->sigma_v = sigma_v;
+  return 0;
+}
+
+// Validation shared by both populate entry points, pop_tracer.c:1144-1160.
+int populate_checks(EZMOCK *ez, const real *params, size_t nexp,
+    size_t *ntracer) {
+  if (!ez) return EZMOCK_ERR_ARG_EZ;
+  if (!params) return EZMOCK_ERR_ARG_PARAMS;
+  if (nexp < EZ_MIN_NUM_TRACER) return EZMOCK_ERR_ARG_LOWEXP;
+  if (!ntracer) return EZMOCK_ERR_ARG_NDATA;
+  Conf *conf = CONF(ez);
+  if (nexp >= (size_t) conf->Ng * conf->Ng * conf->Ng) return EZMOCK_ERR_ARG_HIEXP;
+  Mesh *m = MESH(ez);
+  if (!m->have_field) return EZMOCK_ERR_ARG_MESH;
+  if (COSMO(ez)->vfac == HUGE_VAL) return EZMOCK_ERR_ARG_COSMO;
+  return 0;
+}
+
+}  // namespace
+
+// ============================================================== C ABI ===
+extern "C" {
+
+// EZmock.c:41-124
+EZMOCK *EZmock_init(const double Lbox, const int Ngrid, const int randgen,
+    const uint64_t seed, const int nthread, int *err) {
+  if (!err || *err != EZMOCK_SUCCESS) return NULL;
+  if (Lbox <= 0) { *err = EZMOCK_ERR_ARG_LBOX; return NULL; }
+  if (Ngrid <= 0 || Ngrid > EZ_MAX_GRID) { *err = EZMOCK_ERR_ARG_NGRID; return NULL; }
+  if (randgen != 0 && randgen != 1) { *err = EZMOCK_ERR_ARG_RNG; return NULL; }
+  if (seed == 0) { *err = EZMOCK_ERR_ARG_SEED; return NULL; }
+  if (nthread < 0) { *err = EZMOCK_ERR_ARG_THREAD; return NULL; }
+  EZMOCK *ez = (EZMOCK *) calloc(1, sizeof(EZMOCK));
+  if (!ez) { *err = EZMOCK_ERR_MEMORY; return NULL; }
+  Conf *conf = new (std::nothrow) Conf;
+  RngConf *rng = new (std::nothrow) RngConf;
+  Cosmo *cosmo = new (std::nothrow) Cosmo;
+  Par *par = new (std::nothrow) Par;
+  Mesh *mesh = new (std::nothrow) Mesh;
+  ez->conf = conf; ez->rng = rng; ez->cosmo = cosmo; ez->par = par;
+  ez->mesh = mesh; ez->pk = NULL;
+  if (!conf || !rng || !cosmo || !par || !mesh) {
+    *err = EZMOCK_ERR_MEMORY;
+    EZmock_destroy(ez);
+    return NULL;
+  }
+  conf->nthread = nthread ? nthread : (int) sysconf(_SC_NPROCESSORS_ONLN);
+  conf->Ng = Ngrid;
+  conf->Lbox = Lbox;
+  rng->type = randgen;
+  rng->seed = seed;
+  cosmo->growth2 = cosmo->vfac = HUGE_VAL;
+  par->bao = par->rho_c = par->rho_exp = HUGE_VAL;
+  par->pdf_base = par->sigma_v = HUGE_VAL;
+  return ez;
+}
+
+// EZmock.c:133-142
+void EZmock_destroy(EZMOCK *ez) {
+  if (!ez) return;
+  Mesh *m = MESH(ez);
+  if (m) { free_mesh_device(m); delete m; }
+  delete CONF(ez);
+  delete RNG(ez);
+  delete COSMO(ez);
+  delete PK(ez);
+  delete PAR(ez);
+  free(ez);
+}
+
+// errmsg.c:25-70
+const char *EZmock_errmsg(const int err) {
+  static const char *msg[] = {
+    "no error", "failed to allocate memory", "error code is not set",
+    "EZmock interface uninitialized", "EZmock cosmology uninitialized",
+    "EZmock power spectrum uninitialized",
+    "EZmock density fields uninitialized", "invalid box size",
+    "invalid grid size", "invalid random number generator",
+    "invalid random seed", "invalid number of threads", "null input array",
+    "invalid number of bins", "EZmock calibration parameters not set",
+    "too few expected number of tracers",
+    "too many expected number of tracers",
+    "cannot return the number of tracers", "invalid file name",
+    "invalid RSD factor", "cannot obtain cosmic growth parameters",
+    "infinite value detected", "non-positive value detected",
+    "non-ascending k array detected", "k range insufficient for density field",
+    "too few k bins available for the field",
+    "invalid P(k) after BAO modification",
+    "failed to setup the random generator",
+    "random generator reseting failed",
+    "random generator jumping ahead failed",
+    "invalid cosmological parameters", "invalid density threshold parameter",
+    "invalid exponential bias parameter",
+    "invalid base number for PDF mapping",
+    "invalid parameter for random motions",
+    "the density field has been overwritten",
+    "not enough cells to host all tracers",
+    "PDF implies too many tracers per cell", "too many particles per cell",
+    "failed to initialise FFTW with OpenMP",
+    "failed to write to the output file", "unknown error"};
+  if (err < 0 || err > EZMOCK_ERR_UNKNOWN) return "unregistered error code";
+  return msg[err];
+}
+
+// cosmology.c:53-95 (growth in flat wCDM) and :121-157 (interface)
+int EZmock_set_cosmology(EZMOCK *ez, const double pk_norm, const double fHa,
+    const bool eval, const double z, const double z_pk, const double Omega_m,
+    const double Omega_nu, const double w, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return EZMOCK_ERR_ARG_EZ;      // *err untouched, as upstream
+  Cosmo *cosmo = COSMO(ez);
+  if (!eval) {
+    if (pk_norm <= 0 || fHa <= 0) return (*err = EZMOCK_ERR_PAR_COSMO);
+    cosmo->growth2 = pk_norm;
+    cosmo->vfac = fHa;
+    return EZMOCK_SUCCESS;
+  }
+  if (z < 0 || z_pk < 0 || Omega_m <= 0 || Omega_m > 1 || Omega_nu < 0 ||
+      Omega_nu >= 1 || w >= -1.0 / 3.0) return (*err = EZMOCK_ERR_PAR_COSMO);
+  const double a = 1 / (1 + z), a0 = 1 / (1 + z_pk);
+  if (a <= 0 || a > 1 || a0 <= 0 || a0 > 1) return (*err = EZMOCK_ERR_PAR_COSMO);
+  const double iw = 1 / w;
+  const double c1 = 0.5 * (1 - iw);
+  const double c2 = -0x1.5555555555555p-2 * iw;
+  const double c3 = 1 - 0x1.aaaaaaaaaaaabp-1 * iw;
+  const double x = (1 - 1 / Omega_m) * pow(a, -3 * w);
+  double h1, h2, D, f;
+  if (ez_hyp2f1(c1, c2, c3, x, 1e-12, 10000, &h1) ||
+      ez_hyp2f1(c1 + 1, c2 + 1, c3 + 1, x, 1e-12, 10000, &h2))
+    return (*err = EZMOCK_ERR_COS_GROWTH);
+  f = 1 + 3 * (w - 1) * x / (6 * w - 5) * h2 / h1;
+  if (a == a0) D = 1;
+  else {
+    const double x0 = (1 - 1 / Omega_m) * pow(a0, -3 * w);
+    if (ez_hyp2f1(c1, c2, c3, x0, 1e-12, 10000, &h2))
+      return (*err = EZMOCK_ERR_COS_GROWTH);
+    D = (a * h1) / (a0 * h2);
+  }
+  const double Omn = Omega_m + Omega_nu;
+  const double H = 100 * sqrt((1 - Omn) * pow(a, -3 * (1 + w)) + Omn * pow(a, -3));
+  cosmo->growth2 = D * D;
+  cosmo->vfac = f * H * a;
+  return EZMOCK_SUCCESS;
+}
+
+// linear_pk.c:75-183
+int EZmock_setup_linear_pk(EZMOCK *ez, const double *k, const int n,
+    const double *Pk, const double *Pnw, const double mBAO, const bool logint,
+    int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  if (!k || !Pk || (mBAO != 0 && !Pnw)) return (*err = EZMOCK_ERR_ARG_NULL);
+  if (n <= 3 || n > EZ_MAX_PK_NBIN) return (*err = EZMOCK_ERR_ARG_NBIN);
+  Cosmo *cosmo = COSMO(ez);
+  if (cosmo->growth2 == HUGE_VAL) return (*err = EZMOCK_ERR_ARG_COSMO);
+  for (int i = 0; i < n; i++) {
+    if (!isfinite(k[i]) || !isfinite(Pk[i])) return (*err = EZMOCK_ERR_PK_FINITE);
+    if (k[i] <= 0 || Pk[i] <= 0) return (*err = EZMOCK_ERR_PK_NONPOS);
+    if (i && k[i] <= k[i - 1]) return (*err = EZMOCK_ERR_PK_NONASC);
+  }
+  if (mBAO != 0) {
+    for (int i = 0; i < n; i++) {
+      if (!isfinite(Pnw[i])) return (*err = EZMOCK_ERR_PK_FINITE);
+      if (Pnw[i] <= 0) return (*err = EZMOCK_ERR_PK_NONPOS);
+    }
+  }
+  Conf *conf = CONF(ez);
+  const double kmin = 2 * EZ_PI / conf->Lbox;
+  const double kmax = EZ_PI * conf->Ng / conf->Lbox * sqrt(3);
+  if (k[0] > kmin - EZ_DOUBLE_TOL || k[n - 1] < kmax + EZ_DOUBLE_TOL)
+    return (*err = EZMOCK_ERR_PK_RANGE);
+  int lo, hi;
+  for (lo = 0; lo < n; lo++) if (k[lo] > kmin) break;
+  for (hi = lo; hi < n; hi++) if (k[hi] > kmax) break;
+  lo = (lo < 3) ? 0 : lo - 3;
+  hi = (hi + 3 >= n) ? n : hi + 3;
+  const int nb = hi - lo;
+  if (nb < 3) return (*err = EZMOCK_ERR_PK_NBIN);
+  PkTab *pk = new (std::nothrow) PkTab;
+  if (!pk) return (*err = EZMOCK_ERR_MEMORY);
+  pk->k.resize(nb);
+  pk->P.resize(nb);
+  pk->P2.resize(2 * (size_t) nb);
+  for (int i = 0; i < nb; i++) {
+    const int j = i + lo;
+    if (mBAO == 0) pk->P[i] = Pk[j] * cosmo->growth2;
+    else {
+      pk->P[i] = (k[j] <= EZ_MAX_K_BAO) ?
+          (Pk[j] - Pnw[j]) * exp(k[j] * k[j] * mBAO) + Pnw[j] : Pk[j];
+      if (!isfinite(pk->P[i]) || pk->P[i] <= 0) {
+        delete pk;
+        return (*err = EZMOCK_ERR_PK_MODBAO);
+      }
+      pk->P[i] *= cosmo->growth2;
+    }
+  }
+  for (int i = 0; i < nb; i++) {
+    if (logint) { pk->k[i] = log(k[i + lo]); pk->P[i] = log(pk->P[i]); }
+    else pk->k[i] = k[i + lo];
+  }
+  pk->n = nb;
+  pk->log = logint;
+  pk->kmin = k[lo];
+  pk->kmax = k[hi - 1];
+  ez_cspline_ypp(pk->k.data(), pk->P.data(), nb, pk->P2.data());
+  pk->version = ++g_pk_version;
+  delete PK(ez);
+  ez->pk = pk;
+  PAR(ez)->bao = mBAO;
+  return EZMOCK_SUCCESS;
+}
+
+// dens_field.c:337-432
+int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
+    real *delta, const bool fixamp, const bool iphase, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Conf *conf = CONF(ez);
+  Mesh *m = MESH(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  int e;
+  const bool given = psi && psi[0] && psi[1] && psi[2];
+  if (!given) {
+    PkTab *pk = PK(ez);
+    if (!pk) return (*err = EZMOCK_ERR_ARG_PK);
+    const double kmin = 2 * EZ_PI / conf->Lbox;
+    const double kmax = EZ_PI * conf->Ng / conf->Lbox * sqrt(3);
+    if (pk->kmin > kmin - EZ_DOUBLE_TOL || pk->kmax < kmax + EZ_DOUBLE_TOL)
+      return (*err = EZMOCK_ERR_PK_RANGE);
+    if (!fft_supported(Ng)) {
+      fprintf(stderr, "EZmock: grid size %d is not supported by the GPU FFT "
+          "(supported: 2^n and 3*2^n, 8..2048)\n", Ng);
+      return (*err = EZMOCK_ERR_ARG_NGRID);
+    }
+    if (!delta && RNG(ez)->type != 0) {
+      fprintf(stderr, "EZmock: MT19937 streams are not available on the GPU "
+          "engine; use randgen = 0 (MRG32k3a)\n");
+      return (*err = EZMOCK_ERR_RNG_SET);
+    }
+  }
+  if ((e = device_ready(m))) return (*err = e);
+  m->have_field = false;
+  m->mom_valid = false;
+  m->psi_ref = false;
+  for (int i = 0; i < 3; i++) m->psi_host[i] = nullptr;
+  for (int i = 0; i < 3; i++)
+    if ((e = dalloc(&m->psi[i], N))) return (*err = e);
+  if ((e = dalloc(&m->rho, N))) return (*err = e);
+  if (given) {
+    for (int i = 0; i < 3; i++)
+      HIP_OK(hipMemcpyAsync(m->psi[i], psi[i], N * sizeof(real),
+          hipMemcpyHostToDevice, m->st));
+    if (!deepcopy) {
+      m->psi_ref = true;
+      for (int i = 0; i < 3; i++) m->psi_host[i] = psi[i];
+    }
+  }
+  else {
+    PkTab *pk = PK(ez);
+    const int Ngh = Ng / 2;
+    const size_t plane = (size_t) Ng * Ng;
+    const int nsq = 3 * Ngh * Ngh + 1;
+    double *dk = nullptr, *dP = nullptr, *dP2 = nullptr;
+    if ((e = dalloc(&m->ny, 2 * plane)) || (e = dalloc(&m->ptab, (size_t) nsq)) ||
+        (e = ensure_twiddles(m, Ng)))
+      return (*err = e);
+    if (m->ptab_version != pk->version) {
+      if ((e = dalloc(&dk, pk->n)) || (e = dalloc(&dP, pk->n)) ||
+          (e = dalloc(&dP2, pk->n))) {
+        dfree(&dk); dfree(&dP);
+        return (*err = e);
+      }
+      HIP_OK(hipMemcpyAsync(dk, pk->k.data(), sizeof(double) * pk->n, hipMemcpyHostToDevice, m->st));
+      HIP_OK(hipMemcpyAsync(dP, pk->P.data(), sizeof(double) * pk->n, hipMemcpyHostToDevice, m->st));
+      HIP_OK(hipMemcpyAsync(dP2, pk->P2.data(), sizeof(double) * pk->n, hipMemcpyHostToDevice, m->st));
+      const double kfac = EZ_PI * 2 / conf->Lbox;
+      {
+        Scope s(m, "pk_table");
+        KCHK(launch_pk_table(m->ptab, nsq, dk, dP, dP2, pk->n, pk->log, kfac,
+            log(kfac), m->st));
+      }
+      if ((e = sync_and_collect(m))) return (*err = e);
+      dfree(&dk); dfree(&dP); dfree(&dP2);
+      m->ptab_version = pk->version;
+    }
+    GenParams gp;
+    gp.Ng = Ng;
+    gp.Nh = plane * (Ngh + 1);
+    gp.kfac = EZ_PI * 2 / conf->Lbox;
+    gp.logkfac = log(gp.kfac);
+    gp.Pfac = pow(conf->Lbox, -1.5);
+    gp.logpk = pk->log;
+    gp.fixamp = fixamp;
+    gp.iphase = iphase;
+    mrg_seed_state(RNG(ez)->seed, gp.seed_state);
+    real *spec = nullptr;
+    if (m->keep_spec) {
+      if ((e = dalloc(&m->spec, 2 * gp.Nh))) return (*err = e);
+      spec = m->spec;
+    }
+    cplx *rhok = (cplx *) m->rho;
+    if (delta) {
+      HIP_OK(hipMemcpyAsync(m->psi[0], delta, N * sizeof(real),
+          hipMemcpyHostToDevice, m->st));
+      if ((e = r2c_field(m, Ng, m->psi[0]))) return (*err = e);
+      Scope s(m, "gen_spectrum_wn");
+      KCHK(launch_gen_rhok_wn(rhok, m->ny + plane, (const cplx *) m->psi[0],
+          m->ny, gp, m->ptab, spec, m->st));
+    }
+    else {
+      Scope s(m, "gen_spectrum");
+      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, m->st));
+    }
+    // the generated Nyquist plane lives in ny[plane, 2 plane)
+    cplx *ny_save = m->ny;
+    m->ny = m->ny + plane;
+    for (int c = 0; c < 3; c++) {
+      if ((e = c2r_component(m, Ng, c, m->psi[c]))) { m->ny = ny_save; return (*err = e); }
+    }
+    m->ny = ny_save;
+    if (delta) {
+      const real norm = 1 / sqrt((double) N);
+      Scope s(m, "scale");
+      for (int c = 0; c < 3; c++) KCHK(launch_scale(m->psi[c], norm, N, m->st));
+    }
+  }
+  {
+    Scope s(m, "zero_rho");
+    KCHK(launch_zero(m->rho, N * sizeof(real), m->st));
+  }
+  {
+    Scope s(m, "cic_density");
+    KCHK(launch_cic(m->psi[0], m->psi[1], m->psi[2], m->rho, nullptr, nullptr,
+        Ng, (real) (conf->Ng / conf->Lbox), 0, m->st));
+  }
+  if ((e = sync_and_collect(m))) return (*err = e);
+  m->fixamp = fixamp;
+  m->iphase = iphase;
+  m->have_field = true;
+  return EZMOCK_SUCCESS;
+}
+
+// pop_tracer.c:1140-1219
+int EZmock_populate_tracer(EZMOCK *ez, const real *params, const size_t nexp,
+    const bool att_part, size_t *ntracer, real **x, real **y, real **z,
+    real **vx, real **vy, real **vz, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  int e = populate_checks(ez, params, nexp, ntracer);
+  if (e) return (*err = e);
+  size_t n = 0;
+  if ((e = populate_device(ez, params, nexp, att_part, &n))) return (*err = e);
+  Mesh *m = MESH(ez);
+  real **out[6] = {x, y, z, vx, vy, vz};
+  real *h[6] = {nullptr};
+  for (int i = 0; i < 6; i++) {
+    h[i] = (real *) malloc((n ? n : 1) * sizeof(real));
+    if (!h[i]) {
+      for (int j = 0; j < i; j++) free(h[j]);
+      return (*err = EZMOCK_ERR_MEMORY);
+    }
+  }
+  for (int i = 0; i < 6; i++) {
+    if (hipMemcpyAsync(h[i], m->cat[i], n * sizeof(real), hipMemcpyDeviceToHost,
+        m->st) != hipSuccess) {
+      for (int j = 0; j < 6; j++) free(h[j]);
+      return (*err = EZMOCK_ERR_UNKNOWN);
+    }
+  }
+  if (hipStreamSynchronize(m->st) != hipSuccess) {
+    for (int j = 0; j < 6; j++) free(h[j]);
+    return (*err = EZMOCK_ERR_UNKNOWN);
+  }
+  for (int i = 0; i < 6; i++) *out[i] = h[i];
+  *ntracer = n;
+  return EZMOCK_SUCCESS;
+}
+
+// save_res.c:95-178
+int EZmock_write_ascii(EZMOCK *ez, const real *x, const real *y, const real *z,
+    const real *vx, const real *vy, const real *vz, const size_t ntracer,
+    const double rsd_fac, const bool header, const char *fname, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  // the reference also rejects a first element equal to zero (`!(*x)`)
+  if (!x || !(*x) || !y || !(*y) || !z || !(*z) || !vx || !(*vx) || !vy ||
+      !(*vy) || !vz || !(*vz))
+    return (*err = EZMOCK_ERR_ARG_NULL);
+  if (!ntracer) return (*err = EZMOCK_ERR_ARG_NDATA);
+  Conf *conf = CONF(ez);
+  if (rsd_fac > conf->Lbox) return (*err = EZMOCK_ERR_ARG_RSD);
+  if (!fname || !(*fname)) return (*err = EZMOCK_ERR_ARG_NULL);
+  FILE *fp = fopen(fname, "w");
+  if (!fp) return (*err = EZMOCK_ERR_FILE);
+  const size_t CH = 1048576;
+  std::vector<char> buf(CH);
+  size_t used = 0;
+  auto line = [&](const char *fmt, ...) -> int {
+    va_list ap;
+    va_start(ap, fmt);
+    int n = vsnprintf(buf.data() + used, CH - used, fmt, ap);
+    va_end(ap);
+    if (n < 0 || (size_t) n >= CH) return 1;
+    if (used + n >= CH) {
+      if (fwrite(buf.data(), used, 1, fp) != 1) return 1;
+      used = 0;
+      va_start(ap, fmt);
+      n = vsnprintf(buf.data(), CH, fmt, ap);
+      va_end(ap);
+      if ((size_t) n >= CH) return 1;
+    }
+    used += n;
+    return 0;
+  };
+#ifdef SINGLE_PREC
+#define RF "%.6g"
+#else
+#define RF "%.10g"
+#endif
+  int bad = 0;
+  if (header) {
+    Mesh *m = MESH(ez);
+    Cosmo *c = COSMO(ez);
+    Par *p = PAR(ez);
+    const char *names[2] = {"MRG32K3A", "MT19937"};
+    bad |= line("# BOX_SIZE=%.10lg , NUM_GRID=%d , FIX_AMPLITUDE=%c , "
+        "INVERT_PHASE=%c\n", conf->Lbox, conf->Ng, m->fixamp ? 'T' : 'F',
+        m->iphase ? 'T' : 'F');
+    bad |= line("# GROWTH_PK=%.10lg , VELOCITY_FAC=%.10lg\n", c->growth2, c->vfac);
+    bad |= line("# BAO_ENHANCE=%.10lg , RHO_CRITICAL=%.10lg , RHO_EXP=%.10lg , "
+        "PDF_BASE=%.10lg , SIGMA_VELOCITY=%.10lg\n# RAND_GENERATOR=%s , "
+        "RAND_SEED=%ld\n", p->bao, p->rho_c, p->rho_exp, p->pdf_base,
+        p->sigma_v, names[RNG(ez)->type], (long) RNG(ez)->seed);
+  }
+  if (rsd_fac > 0) {
+    bad |= line("# x(1) y(2) z_rsd(3)\n");
+    for (size_t i = 0; i < ntracer && !bad; i++) {
+      double zr = z[i] + rsd_fac * vz[i];
+      if (zr >= conf->Lbox) zr -= conf->Lbox;
+      if (zr < 0) zr += conf->Lbox;
+      bad |= line(RF " " RF " " RF "\n", x[i], y[i], zr);
+    }
+  }
+  else {
+    bad |= line("# x(1) y(2) z(3) vx(4) vy(5) vz(6)\n");
+    for (size_t i = 0; i < ntracer && !bad; i++)
+      bad |= line(RF " " RF " " RF " " RF " " RF " " RF "\n", x[i], y[i], z[i],
+          vx[i], vy[i], vz[i]);
+  }
+#undef RF
+  if (!bad && fwrite(buf.data(), used, 1, fp) != 1) bad = 1;
+  fclose(fp);
+  return (*err = bad ? EZMOCK_ERR_FILE : EZMOCK_SUCCESS);
+}
+
+// ------------------------------------------------------------ extensions
+int EZmock_ext_real_size(void) { return (int) sizeof(real); }
+
+int EZmock_ext_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int EZmock_ext_populate_device(EZMOCK *ez, const real *params,
+    const size_t nexp, const bool att_part, size_t *ntracer, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  int e = populate_checks(ez, params, nexp, ntracer);
+  if (e) return (*err = e);
+  if ((e = populate_device(ez, params, nexp, att_part, ntracer))) return (*err = e);
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_copy_tracers(EZMOCK *ez, real *x, real *y, real *z, real *vx,
+    real *vy, real *vz, const size_t n, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  if (!m->cat[0] || n > m->ntracer) return (*err = EZMOCK_ERR_ARG_NDATA);
+  real *h[6] = {x, y, z, vx, vy, vz};
+  for (int i = 0; i < 6; i++) {
+    if (!h[i]) continue;
+    if (hipMemcpyAsync(h[i], m->cat[i], n * sizeof(real), hipMemcpyDeviceToHost,
+        m->st) != hipSuccess) return (*err = EZMOCK_ERR_UNKNOWN);
+  }
+  if (hipStreamSynchronize(m->st) != hipSuccess) return (*err = EZMOCK_ERR_UNKNOWN);
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_device_tracers(EZMOCK *ez, void *ptrs[6], size_t *ntracer,
+    int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  for (int i = 0; i < 6; i++) ptrs[i] = m->cat[i];
+  *ntracer = m->ntracer;
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  if (!m->cat[0] || !z_rsd) return (*err = EZMOCK_ERR_ARG_NULL);
+  if (rsd_fac > CONF(ez)->Lbox) return (*err = EZMOCK_ERR_ARG_RSD);
+  const size_t n = m->ntracer;
+  if (m->rsd_cap < n) {
+    dfree(&m->d_rsd);
+    int e = dalloc(&m->d_rsd, n);
+    if (e) return (*err = e);
+    m->rsd_cap = n;
+  }
+  {
+    Scope s(m, "rsd");
+    if (launch_rsd(m->cat[2], m->cat[5], n, rsd_fac, CONF(ez)->Lbox, m->d_rsd,
+        m->st) != hipSuccess) return (*err = EZMOCK_ERR_UNKNOWN);
+  }
+  if (hipMemcpyAsync(z_rsd, m->d_rsd, n * sizeof(double), hipMemcpyDeviceToHost,
+      m->st) != hipSuccess) return (*err = EZMOCK_ERR_UNKNOWN);
+  if (sync_and_collect(m)) return (*err = EZMOCK_ERR_UNKNOWN);
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez || !dst) return (*err = EZMOCK_ERR_ARG_NULL);
+  Mesh *m = MESH(ez);
+  const int Ng = CONF(ez)->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const void *src = nullptr;
+  size_t bytes = 0;
+  switch (which) {
+    case 0: case 1: case 2: src = m->psi[which]; bytes = N * sizeof(real); break;
+    case 3: src = m->rho; bytes = N * sizeof(real); break;
+    case 4: src = m->rhot; bytes = N * sizeof(real); break;
+    case 5: src = m->nt; bytes = N * sizeof(int16_t); break;
+    case 6: src = m->spec; bytes = 2 * (size_t) Ng * Ng * (Ng / 2 + 1) * sizeof(real); break;
+    case 7:
+      memcpy(dst, m->hist_h, sizeof(m->hist_h));
+      return EZMOCK_SUCCESS;
+    default: return (*err = EZMOCK_ERR_ARG_NULL);
+  }
+  if (!src) return (*err = EZMOCK_ERR_ARG_MESH);
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, m->st) != hipSuccess ||
+      hipStreamSynchronize(m->st) != hipSuccess)
+    return (*err = EZMOCK_ERR_UNKNOWN);
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n) {
+  if (!ez || !out || n < 5) return EZMOCK_ERR_ARG_NULL;
+  Mesh *m = MESH(ez);
+  out[0] = m->nmax; out[1] = m->ncell; out[2] = m->ntot;
+  out[3] = m->ncell0; out[4] = m->ntot0;
+  for (size_t i = 0; i < m->pdf.size() && 5 + i < n; i++) out[5 + i] = m->pdf[i];
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
+  if (!ez) return EZMOCK_ERR_ARG_EZ;
+  Mesh *m = MESH(ez);
+  if (option == 1) m->keep_spec = value != 0;
+  else if (option == 2) m->profile = value != 0;
+  else return EZMOCK_ERR_ARG_PARAMS;
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_kernel_times(EZMOCK *ez, double *ms, long *count, const int n,
+    char *names, const size_t cap) {
+  if (!ez) return -1;
+  Mesh *m = MESH(ez);
+  std::string all;
+  int i = 0;
+  for (auto &kv : m->times) {
+    if (i < n) {
+      if (ms) ms[i] = kv.second.first;
+      if (count) count[i] = kv.second.second;
+    }
+    all += kv.first;
+    all += "\n";
+    i++;
+  }
+  if (names && cap) {
+    strncpy(names, all.c_str(), cap - 1);
+    names[cap - 1] = 0;
+  }
+  return i;
+}
+
+void EZmock_ext_reset_kernel_times(EZMOCK *ez) {
+  if (ez) MESH(ez)->times.clear();
+}
+
+int EZmock_ext_sync(EZMOCK *ez, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  if (m->st && sync_and_collect(m)) return (*err = EZMOCK_ERR_UNKNOWN);
+  return EZMOCK_SUCCESS;
+}
+
+// ---- stand-alone routines for the parity tests
+int EZmock_ext_uniform(const int randgen, const uint64_t seed,
+    const uint64_t start, const size_t n, const bool pos, double *out,
+    int *err) {
+  (void) pos;                   // MRG32k3a: get_double == get_double_pos
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err) return *err;
+  if (randgen != 0) return (*err = EZMOCK_ERR_RNG_SET);
+  Mesh tmp;
+  int e = device_ready(&tmp);
+  if (e) return (*err = e);
+  double *d = nullptr;
+  if ((e = dalloc(&d, n))) { free_mesh_device(&tmp); return (*err = e); }
+  MrgState s;
+  mrg_seed_state(seed, s);
+  if (launch_uniform(s, start, n, tmp.jtab, d, tmp.st) != hipSuccess ||
+      hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
+      hipStreamSynchronize(tmp.st) != hipSuccess)
+    e = EZMOCK_ERR_UNKNOWN;
+  dfree(&d);
+  free_mesh_device(&tmp);
+  return (*err = e);
+}
+
+static int fft_test(const int Ng, const real *in, real *out, int sign, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err) return *err;
+  if (!fft_supported(Ng)) return (*err = EZMOCK_ERR_ARG_NGRID);
+  Mesh tmp;
+  int e = device_ready(&tmp);
+  if (e) return (*err = e);
+  const size_t N = (size_t) Ng * Ng * Ng, plane = (size_t) Ng * Ng;
+  const int Ngh = Ng / 2;
+  const size_t nh = plane * (Ngh + 1);
+  real *buf = nullptr, *half = nullptr;
+  if ((e = dalloc(&buf, N)) || (e = dalloc(&half, 2 * nh)) ||
+      (e = dalloc(&tmp.ny, 2 * plane)) || (e = dalloc(&tmp.rho, N)) ||
+      (e = ensure_twiddles(&tmp, Ng))) {
+    dfree(&buf); dfree(&half); free_mesh_device(&tmp);
+    return (*err = e);
+  }
+  if (sign > 0) {
+    // reference layout in -> generator layout (rhok + Nyquist plane)
+    std::vector<real> main(N), nyq(2 * plane);
+    for (size_t ij = 0; ij < plane; ij++) {
+      for (int k = 0; k < Ngh; k++) {
+        main[2 * (ij * Ngh + k)] = in[2 * (ij * (Ngh + 1) + k)];
+        main[2 * (ij * Ngh + k) + 1] = in[2 * (ij * (Ngh + 1) + k) + 1];
+      }
+      nyq[2 * ij] = in[2 * (ij * (Ngh + 1) + Ngh)];
+      nyq[2 * ij + 1] = in[2 * (ij * (Ngh + 1) + Ngh) + 1];
+    }
+    e = 0;
+    if (hipMemcpy(tmp.rho, main.data(), N * sizeof(real), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(tmp.ny, nyq.data(), 2 * plane * sizeof(real), hipMemcpyHostToDevice) != hipSuccess)
+      e = EZMOCK_ERR_UNKNOWN;
+    // comp 1 multiplies by n_y: use a dedicated "identity" component instead
+    if (!e) {
+      cplx *b = (cplx *) buf;
+      if (fft_strided_pass(b, (const cplx *) tmp.rho, tmp.ny, tmp.twN, Ng, 0, +1, 1, 3, tmp.st) != hipSuccess ||
+          fft_strided_pass(b, nullptr, nullptr, tmp.twN, Ng, 1, +1, 0, 0, tmp.st) != hipSuccess ||
+          fft_row_pass(b, tmp.twN, tmp.twh, Ng, +1, tmp.st) != hipSuccess ||
+          hipMemcpyAsync(out, buf, N * sizeof(real), hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
+          hipStreamSynchronize(tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+    }
+  }
+  else {
+    e = 0;
+    if (hipMemcpy(buf, in, N * sizeof(real), hipMemcpyHostToDevice) != hipSuccess ||
+        r2c_field(&tmp, Ng, buf) ||
+        hipStreamSynchronize(tmp.st) != hipSuccess)
+      e = EZMOCK_ERR_UNKNOWN;
+    if (!e) {
+      std::vector<real> main(N), nyq(2 * plane);
+      if (hipMemcpy(main.data(), buf, N * sizeof(real), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(nyq.data(), tmp.ny, 2 * plane * sizeof(real), hipMemcpyDeviceToHost) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+      for (size_t ij = 0; ij < plane && !e; ij++) {
+        for (int k = 0; k < Ngh; k++) {
+          out[2 * (ij * (Ngh + 1) + k)] = main[2 * (ij * Ngh + k)];
+          out[2 * (ij * (Ngh + 1) + k) + 1] = main[2 * (ij * Ngh + k) + 1];
+        }
+        out[2 * (ij * (Ngh + 1) + Ngh)] = nyq[2 * ij];
+        out[2 * (ij * (Ngh + 1) + Ngh) + 1] = nyq[2 * ij + 1];
+      }
+    }
+  }
+  dfree(&buf); dfree(&half);
+  free_mesh_device(&tmp);
+  return (*err = e);
+}
+
+int EZmock_ext_fft_c2r(const int Ng, const real *in, real *out, int *err) {
+  return fft_test(Ng, in, out, +1, err);
+}
+
+int EZmock_ext_fft_r2c(const int Ng, const real *in, real *out, int *err) {
+  return fft_test(Ng, in, out, -1, err);
+}
+
+int EZmock_ext_pdf_walk(const real *rhot, const size_t num, const size_t ncell,
+    const size_t *pdf, const int nmax, int16_t *nt, size_t *pdf_out,
+    size_t *ncell_out, size_t *ntot_out, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err) return *err;
+  if (!rhot || !pdf || !nt || nmax < 1 || nmax > EZ_MAX_NUM_TRACER_PER_CELL)
+    return (*err = EZMOCK_ERR_ARG_NULL);
+  Mesh m;
+  int e = device_ready(&m);
+  if (e) return (*err = e);
+  // histogram on the host (64 bins, pop_tracer.c:265-268 with ilogb_max 5)
+  unsigned long long hist[EZ_NUM_LOGB_BIN] = {0};
+  for (size_t i = 0; i < num; i++) {
+    if (rhot[i] == 0) continue;
+    int d = (int) ((unsigned) 5 - (unsigned) ilogb(rhot[i]));
+    if (d < 0) d = 0;
+    else if (d >= EZ_NUM_LOGB_BIN) d = EZ_NUM_LOGB_BIN - 1;
+    hist[d]++;
+  }
+  int imax;
+  size_t cnt = 0;
+  for (imax = 0; imax < EZ_NUM_LOGB_BIN; imax++) {
+    cnt += hist[imax];
+    if (cnt >= ncell) break;
+  }
+  if (imax == EZ_NUM_LOGB_BIN) { free_mesh_device(&m); return (*err = EZMOCK_ERR_UNKNOWN); }
+  const real thres = (imax == EZ_NUM_LOGB_BIN - 1) ? (real) EZ_REAL_MIN : (real) pow(2, 5 - imax);
+  const size_t hw = sort_hist_words(cnt + 1);
+  real *d_rhot = nullptr;
+  if ((e = dalloc(&d_rhot, num)) || (e = dalloc(&m.keys, cnt + 1)) ||
+      (e = dalloc(&m.keys_tmp, cnt + 1)) || (e = dalloc(&m.sort_gh, hw)) ||
+      (e = dalloc(&m.sort_go, hw)) || (e = dalloc(&m.sort_scan, scan_tmp_words(hw) + 8)) ||
+      (e = dalloc(&m.counter, 2)) || (e = dalloc(&m.d_num, nmax)) ||
+      (e = dalloc(&m.d_cnt, nmax)) || (e = dalloc(&m.d_vb, nmax)) ||
+      (e = dalloc(&m.d_res, 4)) || (e = dalloc(&m.nt, num)) ||
+      (e = dalloc(&m.blockcnt, (num + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK))) {
+    dfree(&d_rhot); free_mesh_device(&m);
+    return (*err = e);
+  }
+  std::vector<uint64_t> numu(pdf, pdf + nmax);
+  uint64_t res[3] = {0, 0, 0}, ncand = 0;
+  std::vector<uint64_t> pcnt(nmax);
+  e = 0;
+  if (hipMemcpy(d_rhot, rhot, num * sizeof(real), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m.d_num, numu.data(), nmax * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_zero(m.counter, 16, m.st) != hipSuccess ||
+      launch_compact(d_rhot, thres, num, m.keys, cnt + 1, m.counter, m.st) != hipSuccess ||
+      launch_sort(m.keys, m.keys_tmp, cnt, m.sort_gh, m.sort_go, m.sort_scan, m.st) != hipSuccess ||
+      launch_walk(m.keys, cnt, m.d_num, nmax, m.d_vb, m.d_cnt, m.d_res, m.st) != hipSuccess ||
+      launch_assign_nt(d_rhot, num, m.d_vb, nmax, m.nt, m.blockcnt, m.st) != hipSuccess ||
+      hipMemcpyAsync(res, m.d_res, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, m.st) != hipSuccess ||
+      hipMemcpyAsync(&ncand, m.counter, sizeof(uint64_t), hipMemcpyDeviceToHost, m.st) != hipSuccess ||
+      hipMemcpyAsync(pcnt.data(), m.d_cnt, nmax * sizeof(uint64_t), hipMemcpyDeviceToHost, m.st) != hipSuccess ||
+      hipMemcpyAsync(nt, m.nt, num * sizeof(int16_t), hipMemcpyDeviceToHost, m.st) != hipSuccess ||
+      hipStreamSynchronize(m.st) != hipSuccess)
+    e = EZMOCK_ERR_UNKNOWN;
+  if (!e && (ncand != cnt || res[2])) e = EZMOCK_ERR_UNKNOWN;
+  if (!e) {
+    for (int i = 0; i < nmax; i++) pdf_out[i] = pcnt[i];
+    *ncell_out = res[0];
+    *ntot_out = res[1];
+  }
+  dfree(&d_rhot);
+  free_mesh_device(&m);
+  return (*err = e);
+}
+
+}  // extern "C"

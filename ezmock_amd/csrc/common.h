@@ -1,0 +1,73 @@
+// common.h -- shared definitions of the MI355X EZmock engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <stdio.h>
+
+#include "EZmock.h"
+
+// Error codes: numbering of the reference's errmsg.h:33-88 (OpenMP build).
+enum {
+  EZMOCK_SUCCESS = 0, EZMOCK_ERR_MEMORY, EZMOCK_ERR_ARG_ECODE,
+  EZMOCK_ERR_ARG_EZ, EZMOCK_ERR_ARG_COSMO, EZMOCK_ERR_ARG_PK,
+  EZMOCK_ERR_ARG_MESH, EZMOCK_ERR_ARG_LBOX, EZMOCK_ERR_ARG_NGRID,
+  EZMOCK_ERR_ARG_RNG, EZMOCK_ERR_ARG_SEED, EZMOCK_ERR_ARG_THREAD,
+  EZMOCK_ERR_ARG_NULL, EZMOCK_ERR_ARG_NBIN, EZMOCK_ERR_ARG_PARAMS,
+  EZMOCK_ERR_ARG_LOWEXP, EZMOCK_ERR_ARG_HIEXP, EZMOCK_ERR_ARG_NDATA,
+  EZMOCK_ERR_ARG_FNAME, EZMOCK_ERR_ARG_RSD, EZMOCK_ERR_COS_GROWTH,
+  EZMOCK_ERR_PK_FINITE, EZMOCK_ERR_PK_NONPOS, EZMOCK_ERR_PK_NONASC,
+  EZMOCK_ERR_PK_RANGE, EZMOCK_ERR_PK_NBIN, EZMOCK_ERR_PK_MODBAO,
+  EZMOCK_ERR_RNG_SET, EZMOCK_ERR_RNG_RESET, EZMOCK_ERR_RNG_JUMP,
+  EZMOCK_ERR_PAR_COSMO, EZMOCK_ERR_PAR_RCUT, EZMOCK_ERR_PAR_REXP,
+  EZMOCK_ERR_PAR_PBASE, EZMOCK_ERR_PAR_SIGV, EZMOCK_ERR_RHO_USED,
+  EZMOCK_ERR_RHO_LESS, EZMOCK_ERR_RHO_PBASE, EZMOCK_ERR_RHO_NPART,
+  EZMOCK_ERR_FFT_INIT, EZMOCK_ERR_FILE, EZMOCK_ERR_UNKNOWN
+};
+
+// Model and configuration constants (reference config.h:39-92).
+#define EZ_PI 3.14159265358979323846
+#define EZ_DOUBLE_TOL 1e-8
+#define EZ_MAX_GRID 65536
+#define EZ_MAX_PK_NBIN 1073741824
+#define EZ_MIN_NUM_TRACER 128
+#define EZ_MAX_NUM_TRACER_PER_CELL 1024
+#define EZ_MAX_K_BAO 0.5
+#define EZ_NUM_LOGB_BIN 64
+#define EZ_RHO_SAT 100
+#define EZ_GAUSS_WIDTH 10
+
+#ifdef SINGLE_PREC
+#define EZ_REAL_MIN 1.17549435082228750797e-38F
+#define EZ_REAL_TOL 1e-5
+typedef float2 cplx;
+typedef uint32_t rkey;          // order-preserving key of a positive real
+#else
+#define EZ_REAL_MIN 2.2250738585072013830902e-308
+#define EZ_REAL_TOL 1e-10
+typedef double2 cplx;
+typedef uint64_t rkey;
+#endif
+
+// Random-stream positions of the reference (draw units; N = Ng^3):
+//   noise: cell idx >= 1 of the half-complex grid -> (idx-1)*2 (amp, phase)
+//          or (idx-1) with fixed amplitudes   perturb.c:112-118,153-164
+//   bias pair p                  -> 2N + 2p   pop_tracer.c:233-237,256
+//   attach-to-particle draws     -> 4N        pop_tracer.c:781
+//   tracer t position            -> 5N + 3t   pop_tracer.c:919
+//   velocity scatter, pair q     -> 8N + 6q   pop_tracer.c:1071
+
+#define HIP_OK(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      fprintf(stderr, "EZmock: HIP error %s at %s:%d\n",                    \
+              hipGetErrorString(e_), __FILE__, __LINE__);                   \
+      return EZMOCK_ERR_UNKNOWN;                                            \
+    }                                                                       \
+  } while (0)
+
+static inline unsigned ez_blocks(size_t n, unsigned per) {
+  return (unsigned) ((n + per - 1) / per);
+}

@@ -1,0 +1,19 @@
+// fft.h -- gfx950 3-D real FFT passes (see fft.hip for the data layout).
+#pragma once
+#include "common.h"
+
+// Supported grid sizes: 2^n and 3*2^n from 8 to 2048.
+bool fft_supported(int Ng);
+
+// One strided c2c pass along axis 0 (i) or 1 (j) of the packed half-complex
+// field buf[Ng][Ng][Ng/2].  sign +1: e^{+ikx}, -1: e^{-ikx}.  mode 1 reads
+// the generated spectrum (rhok, rhok_ny) instead of buf and applies the
+// component multiply `comp` (0 x, 1 y, 2 z) and kz = 0 / Nyquist packing.
+hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
+    const cplx *tw, int Ng, int axis, int sign, int mode, int comp,
+    hipStream_t st);
+
+// Row pass along kk: sign +1 = c2r (packed half spectrum -> Ng reals per
+// row), -1 = r2c.  twN: exp(+2 pi i m / Ng), twh: exp(+2 pi i m / (Ng/2)).
+hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
+    int sign, hipStream_t st);

@@ -1,0 +1,470 @@
+// fft.hip -- 3-D real FFTs for gfx950, replacing the reference's FFTW plans
+// (fftw_def.h:29-64; c2r executed at perturb.c:361,397,398,556,592,593,
+// r2c at dens_field.c:197-201).
+//
+// Data layout (device, complex `cplx`):  C[i][j][kk], kk in [0, Ng/2), i.e.
+// exactly Ng^3 reals -- the same bytes as the real-space output, so every
+// transform runs in place in the output buffer.  The kz = 0 and kz = Ng/2
+// planes of the half spectrum are both Hermitian in (i, j) after the
+// c2r's implicit projection, so they are packed into the kk = 0 slot as
+// P = H(kz=0) + i H(kz=Ny); after the (i, j) transforms the real part of the
+// slot is the kz=0 column and the imaginary part the Nyquist column.
+//
+// Passes (each reads and writes the field once, HBM-bound):
+//   strided pass, axis i or j: one workgroup = TK consecutive kk columns
+//     (128-byte rows) x all L points, staged in LDS; Stockham radix-8/4/2/3
+//     stages in LDS with register staging; optionally fused with the
+//     spectrum multiply of the first pass (MODE 1).
+//   row pass (c2r): TR contiguous rows; Hermitian pre-processing
+//     Z[m] = X[m] + conj(X[L/2-m]) + i W^m (X[m] - conj(X[L/2-m])), an L/2
+//     complex FFT, and the row of L reals is the complex result.
+#include "common.h"
+#include "fft.h"
+
+// ------------------------------------------------------------ radix plan
+__host__ __device__ constexpr int pow2_exp(int L) {
+  int e = 0;
+  while (L % 2 == 0) { L /= 2; e++; }
+  return e;
+}
+__host__ __device__ constexpr int pow3_exp(int L) {
+  int e = 0;
+  while (L % 3 == 0) { L /= 3; e++; }
+  return e;
+}
+// Radix sequence: radix-8 stages, the power-of-two remainder as 4 or 4*4,
+// then radix-3 stages.
+__host__ __device__ constexpr int n_stages(int L) {
+  // rem 1: one radix-8 stage becomes 4*4 (or a lone radix-2); rem 2: a 4.
+  const int e = pow2_exp(L), n8 = e / 3, rem = e % 3;
+  return n8 + (rem != 0 ? 1 : 0) + pow3_exp(L);
+}
+__host__ __device__ constexpr int radix_at(int L, int s) {
+  int e = pow2_exp(L), n8 = e / 3, rem = e % 3;
+  int seq[40] = {0};
+  int n = 0;
+  if (rem == 1 && n8 > 0) {
+    for (int i = 0; i < n8 - 1; i++) seq[n++] = 8;
+    seq[n++] = 4; seq[n++] = 4;
+  }
+  else {
+    for (int i = 0; i < n8; i++) seq[n++] = 8;
+    if (rem == 1) seq[n++] = 2;
+    if (rem == 2) seq[n++] = 4;
+  }
+  for (int i = 0; i < pow3_exp(L); i++) seq[n++] = 3;
+  return seq[s];
+}
+__host__ __device__ constexpr int prefix_prod(int L, int s) {
+  int p = 1;
+  for (int i = 0; i < s; i++) p *= radix_at(L, i);
+  return p;
+}
+
+// ------------------------------------------------------- complex helpers
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <class C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by SIGN * i
+template <class C, int SIGN> __device__ __forceinline__ C cmuli(C a) {
+  return (SIGN > 0) ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+template <class C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+
+// In-register DFT of size R, X[q] = sum_t x[t] exp(SIGN 2 pi i q t / R).
+template <class C, int R, int SIGN> struct Dft;
+
+template <class C, int SIGN> struct Dft<C, 2, SIGN> {
+  __device__ __forceinline__ static void run(C *x) {
+    C a = x[0], b = x[1];
+    x[0] = cadd(a, b); x[1] = csub(a, b);
+  }
+};
+template <class C, int SIGN> struct Dft<C, 4, SIGN> {
+  __device__ __forceinline__ static void run(C *x) {
+    C a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    C c = cadd(x[1], x[3]), d = cmuli<C, SIGN>(csub(x[1], x[3]));
+    x[0] = cadd(a, c); x[2] = csub(a, c);
+    x[1] = cadd(b, d); x[3] = csub(b, d);
+  }
+};
+template <class C, int SIGN> struct Dft<C, 8, SIGN> {
+  __device__ __forceinline__ static void run(C *x) {
+    using R = decltype(x[0].x);
+    const R h = (R) 0.70710678118654752440;
+    C e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+    Dft<C, 4, SIGN>::run(e);
+    Dft<C, 4, SIGN>::run(o);
+    // twiddles w8^(SIGN q), q = 1..3
+    C o1 = cmul(o[1], C{h, (R) SIGN * h});
+    C o2 = cmuli<C, SIGN>(o[2]);
+    C o3 = cmul(o[3], C{-h, (R) SIGN * h});
+    x[0] = cadd(e[0], o[0]); x[4] = csub(e[0], o[0]);
+    x[1] = cadd(e[1], o1);   x[5] = csub(e[1], o1);
+    x[2] = cadd(e[2], o2);   x[6] = csub(e[2], o2);
+    x[3] = cadd(e[3], o3);   x[7] = csub(e[3], o3);
+  }
+};
+template <class C, int SIGN> struct Dft<C, 3, SIGN> {
+  __device__ __forceinline__ static void run(C *x) {
+    using R = decltype(x[0].x);
+    const R s3 = (R) 0.86602540378443864676;
+    C a = x[0], b = x[1], c = x[2];
+    C t = cadd(b, c);
+    C m = C{a.x - (R) 0.5 * t.x, a.y - (R) 0.5 * t.y};
+    C d = csub(b, c);
+    C r = cmuli<C, SIGN>(C{s3 * d.x, s3 * d.y});
+    x[0] = cadd(a, t);
+    x[1] = cadd(m, r);
+    x[2] = csub(m, r);
+  }
+};
+
+// ----------------------------------------------------------- LDS stages
+// NCOL transforms of length L; element (n, col) at lds[n*SN + col*SC].
+// COLFAST: consecutive threads take consecutive columns (SC == 1 layouts).
+template <class C, int L, int NCOL, int NT, int SIGN, int SN, int SC,
+          bool COLFAST, int S>
+__device__ __forceinline__ void lds_stage(C *lds, const C *__restrict__ tw) {
+  constexpr int R = radix_at(L, S);
+  constexpr int Ls = prefix_prod(L, S);
+  constexpr int r = L / Ls, rp = r / R;
+  constexpr int NBC = L / R;
+  constexpr int NB = NCOL * NBC;
+  constexpr int IT = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  C v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int b = tid + it * NT;
+    if (IT * NT == NB || b < NB) {
+      const int col = COLFAST ? b % NCOL : b / NBC;
+      const int bb = COLFAST ? b / NCOL : b % NBC;
+      const int j = bb / rp, k = bb % rp;
+#pragma unroll
+      for (int t = 0; t < R; t++) v[it][t] = lds[(j * r + k + rp * t) * SN + col * SC];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int b = tid + it * NT;
+    if (IT * NT == NB || b < NB) {
+      const int col = COLFAST ? b % NCOL : b / NBC;
+      const int bb = COLFAST ? b / NCOL : b % NBC;
+      const int j = bb / rp, k = bb % rp;
+      if (Ls > 1) {
+#pragma unroll
+        for (int t = 1; t < R; t++) {
+          C w = tw[j * t * rp];
+          if (SIGN < 0) w.y = -w.y;
+          v[it][t] = cmul(v[it][t], w);
+        }
+      }
+      Dft<C, R, SIGN>::run(v[it]);
+#pragma unroll
+      for (int q = 0; q < R; q++) lds[((j + Ls * q) * rp + k) * SN + col * SC] = v[it][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <class C, int L, int NCOL, int NT, int SIGN, int SN, int SC,
+          bool COLFAST, int S = 0>
+__device__ __forceinline__ void lds_fft(C *lds, const C *__restrict__ tw) {
+  if constexpr (S < n_stages(L)) {
+    lds_stage<C, L, NCOL, NT, SIGN, SN, SC, COLFAST, S>(lds, tw);
+    lds_fft<C, L, NCOL, NT, SIGN, SN, SC, COLFAST, S + 1>(lds, tw);
+  }
+}
+
+// ------------------------------------------------------- strided passes
+// Spectrum multiply of the first c2r pass (axis i): psi_a(k) = rhok(k) * n_a
+// with the c2r projection on the kz = 0 and kz = Ny planes (lines where the
+// pair is anti-Hermitian vanish: i = Ng/2 for a = x, j = Ng/2 for a = y, the
+// whole Nyquist plane for a = z) and the packing P = H0 + i HNy at kk = 0.
+template <class C>
+__device__ __forceinline__ C spec_mul(C h, int comp, int ki, int kj, int kz) {
+  using R = decltype(C{}.x);
+  const R n = (comp == 0) ? (R) ki : (comp == 1) ? (R) kj :
+              (comp == 2) ? (R) kz : (R) 1;
+  return C{h.x * n, h.y * n};
+}
+
+// kz = 0 and kz = Ny planes: Hermitian part (H(k) + conj H(-k)) / 2 of each
+// plane (exact for the generator's mirrored modes; anti-Hermitian lines of
+// psi_x, psi_y and the psi_z Nyquist plane vanish), packed as H0 + i HNy.
+// comp 3 = unit multiplier (plain c2r of rhok).
+template <class C>
+__device__ __forceinline__ C spec_load(const C *__restrict__ rhok,
+    const C *__restrict__ rhok_ny, int i, int j, int kk, int Ng, int Ngh,
+    int comp) {
+  using R = decltype(C{}.x);
+  const int ki = (i <= Ngh) ? i : i - Ng;
+  const int kj = (j <= Ngh) ? j : j - Ng;
+  if (kk > 0) return spec_mul(rhok[((size_t) i * Ng + j) * Ngh + kk], comp, ki, kj, kk);
+  const int ni = (i == 0) ? 0 : Ng - i, nj = (j == 0) ? 0 : Ng - j;
+  const int kni = (ni <= Ngh) ? ni : ni - Ng, knj = (nj <= Ngh) ? nj : nj - Ng;
+  const size_t p = (size_t) i * Ng + j, q = (size_t) ni * Ng + nj;
+  const C a0 = spec_mul(rhok[p * Ngh], comp, ki, kj, 0);
+  const C b0 = spec_mul(rhok[q * Ngh], comp, kni, knj, 0);
+  const C a1 = spec_mul(rhok_ny[p], comp, ki, kj, Ngh);
+  const C b1 = spec_mul(rhok_ny[q], comp, kni, knj, Ngh);
+  const C h0 = C{(a0.x + b0.x) * (R) 0.5, (a0.y - b0.y) * (R) 0.5};
+  const C hn = C{(a1.x + b1.x) * (R) 0.5, (a1.y - b1.y) * (R) 0.5};
+  return C{h0.x - hn.y, h0.y + hn.x};
+}
+
+// MODE 0: in place on buf; MODE 1: first pass from the generated spectrum.
+// Grid: x = kk tile, y = outer line.  The transform index n walks `stride`.
+template <class C, int L, int TK, int NT, int SIGN, int MODE>
+__global__ void __launch_bounds__(NT)
+k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
+    const C *__restrict__ rhok_ny, const C *__restrict__ tw, int Ng, int Ngh,
+    size_t stride, size_t outer_stride, int comp) {
+  __shared__ C lds[L * TK];
+  const int kk0 = blockIdx.x * TK;
+  const int o = blockIdx.y;
+  const size_t base = (size_t) o * outer_stride + kk0;
+  constexpr int E = L * TK;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) {
+    const int n = e / TK, c = e % TK;
+    C v;
+    if (MODE == 0) v = buf[base + (size_t) n * stride + c];
+    else v = spec_load(rhok, rhok_ny, n, o, kk0 + c, Ng, Ngh, comp);
+    lds[n * TK + c] = v;
+  }
+  __syncthreads();
+  lds_fft<C, L, TK, NT, SIGN, TK, 1, true>(lds, tw);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) {
+    const int n = e / TK, c = e % TK;
+    buf[base + (size_t) n * stride + c] = lds[n * TK + c];
+  }
+}
+
+// ------------------------------------------------------------ row passes
+// c2r along kk: in place, rows of Lh complex -> 2 Lh reals.
+// twN[m] = exp(+2 pi i m / (2 Lh)), twh[m] = exp(+2 pi i m / Lh).
+template <class C, int Lh, int TR, int NT>
+__global__ void __launch_bounds__(NT)
+k_fft_rows_c2r(C *__restrict__ buf, const C *__restrict__ twN,
+    const C *__restrict__ twh) {
+  using R = decltype(C{}.x);
+  __shared__ C lds[Lh * TR];
+  const size_t row0 = (size_t) blockIdx.x * TR;
+  C *g = buf + row0 * Lh;
+  constexpr int E = Lh * TR;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) lds[e] = g[e];
+  __syncthreads();
+  // Hermitian pre-processing, pairs (m, Lh - m) for m in [0, Lh/2].
+  constexpr int NP = Lh / 2 + 1;
+  constexpr int NPT = NP * TR;
+  constexpr int IT = (NPT + NT - 1) / NT;
+  C za[IT], zb[IT];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int p = threadIdx.x + it * NT;
+    if (p < NPT) {
+      const int rr = p / NP, m = p % NP;
+      C *row = lds + rr * Lh;
+      if (m == 0) {
+        const C s = row[0];                 // (G0, GNy)
+        za[it] = C{s.x + s.y, s.x - s.y};
+      }
+      else {
+        const int m2 = Lh - m;
+        const C xm = row[m], xn = row[m2];
+        const C wm = twN[m], wn = twN[m2];
+        // Z[m]  = X[m]  + conj(X[m2]) + i W^m  (X[m]  - conj(X[m2]))
+        // Z[m2] = X[m2] + conj(X[m])  + i W^m2 (X[m2] - conj(X[m]))
+        C s1 = cadd(xm, cconj(xn)), d1 = csub(xm, cconj(xn));
+        C s2 = cadd(xn, cconj(xm)), d2 = csub(xn, cconj(xm));
+        C t1 = cmul(wm, d1), t2 = cmul(wn, d2);
+        za[it] = C{s1.x - t1.y, s1.y + t1.x};
+        zb[it] = C{s2.x - t2.y, s2.y + t2.x};
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int p = threadIdx.x + it * NT;
+    if (p < NPT) {
+      const int rr = p / NP, m = p % NP;
+      C *row = lds + rr * Lh;
+      row[m] = za[it];
+      if (m != 0 && m != Lh - m) row[Lh - m] = zb[it];
+    }
+  }
+  __syncthreads();
+  (void) sizeof(R);
+  lds_fft<C, Lh, TR, NT, +1, 1, Lh, false>(lds, twh);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) g[e] = lds[e];
+}
+
+// r2c along kk (forward): rows of 2 Lh reals viewed as Lh complex z[n] =
+// y[2n] + i y[2n+1]; after the Lh-point forward FFT Z[m]:
+//   X[m] = (Z[m] + conj(Z[Lh-m]))/2 - i/2 W^-m (Z[m] - conj(Z[Lh-m])),
+// X[0] and X[Lh] are real; they are packed as X[0] + i X[Lh] in slot 0.
+template <class C, int Lh, int TR, int NT>
+__global__ void __launch_bounds__(NT)
+k_fft_rows_r2c(C *__restrict__ buf, const C *__restrict__ twN,
+    const C *__restrict__ twh) {
+  using R = decltype(C{}.x);
+  __shared__ C lds[Lh * TR];
+  const size_t row0 = (size_t) blockIdx.x * TR;
+  C *g = buf + row0 * Lh;
+  constexpr int E = Lh * TR;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) lds[e] = g[e];
+  __syncthreads();
+  lds_fft<C, Lh, TR, NT, -1, 1, Lh, false>(lds, twh);
+  constexpr int NP = Lh / 2 + 1;
+  constexpr int NPT = NP * TR;
+  constexpr int IT = (NPT + NT - 1) / NT;
+  C xa[IT], xb[IT];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int p = threadIdx.x + it * NT;
+    if (p < NPT) {
+      const int rr = p / NP, m = p % NP;
+      C *row = lds + rr * Lh;
+      if (m == 0) {
+        const C z = row[0];
+        xa[it] = C{z.x + z.y, z.x - z.y};   // (X[0], X[Lh]) packed
+      }
+      else {
+        const int m2 = Lh - m;
+        const C zm = row[m], zn = row[m2];
+        C wm = twN[m], wn = twN[m2];
+        wm.y = -wm.y; wn.y = -wn.y;          // W^-m
+        C s1 = cadd(zm, cconj(zn)), d1 = csub(zm, cconj(zn));
+        C s2 = cadd(zn, cconj(zm)), d2 = csub(zn, cconj(zm));
+        C t1 = cmul(wm, d1), t2 = cmul(wn, d2);
+        // X = s/2 - i/2 * t
+        xa[it] = C{(R) 0.5 * (s1.x + t1.y), (R) 0.5 * (s1.y - t1.x)};
+        xb[it] = C{(R) 0.5 * (s2.x + t2.y), (R) 0.5 * (s2.y - t2.x)};
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int p = threadIdx.x + it * NT;
+    if (p < NPT) {
+      const int rr = p / NP, m = p % NP;
+      C *row = lds + rr * Lh;
+      row[m] = xa[it];
+      if (m != 0 && m != Lh - m) row[Lh - m] = xb[it];
+    }
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int e = threadIdx.x; e < E; e += NT) g[e] = lds[e];
+}
+
+// --------------------------------------------------------- dispatching
+template <int L> struct StridedCfg {
+#ifdef SINGLE_PREC
+  static constexpr int TKMAX = (L <= 1024) ? 16 : 8;
+#else
+  static constexpr int TKMAX = (L <= 1024) ? 8 : 4;
+#endif
+};
+template <int L> struct RowCfg {
+#ifdef SINGLE_PREC
+  static constexpr int TR = (L <= 512) ? 16 : 8;
+#else
+  static constexpr int TR = (L <= 512) ? 8 : 4;
+#endif
+  static constexpr int E = L * TR;
+  static constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
+};
+
+template <int L, int TK, int SIGN>
+static hipError_t launch_strided_tk(cplx *buf, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, int axis, int mode,
+    int comp, hipStream_t st) {
+  constexpr int E = L * TK;
+  constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
+  const size_t plane = (size_t) Ng * Ngh;
+  const size_t stride = (axis == 0) ? plane : (size_t) Ngh;
+  const size_t outer = (axis == 0) ? (size_t) Ngh : plane;
+  dim3 grid(Ngh / TK, Ng);
+  if (mode == 0)
+    hipLaunchKernelGGL((k_fft_strided<cplx, L, TK, NT, SIGN, 0>), grid, dim3(NT), 0, st,
+        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp);
+  else
+    hipLaunchKernelGGL((k_fft_strided<cplx, L, TK, NT, SIGN, 1>), grid, dim3(NT), 0, st,
+        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp);
+  return hipGetLastError();
+}
+
+template <int L, int SIGN>
+static hipError_t launch_strided(cplx *buf, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, int axis, int mode,
+    int comp, hipStream_t st) {
+  constexpr int T = StridedCfg<L>::TKMAX;
+  if (Ngh % T == 0) return launch_strided_tk<L, T, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+  if constexpr (T >= 8) if (Ngh % (T / 2) == 0) return launch_strided_tk<L, T / 2, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+  if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_strided_tk<L, 4, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+  if (Ngh % 2 == 0) return launch_strided_tk<L, 2, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+  return launch_strided_tk<L, 1, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+}
+
+template <int Lh>
+static hipError_t launch_rows(cplx *buf, const cplx *twN, const cplx *twh,
+    size_t nrows, int sign, hipStream_t st) {
+  constexpr int TR = RowCfg<Lh>::TR, NT = RowCfg<Lh>::NT;
+  if (nrows % TR) return hipErrorInvalidValue;
+  dim3 grid((unsigned) (nrows / TR));
+  if (sign > 0)
+    hipLaunchKernelGGL((k_fft_rows_c2r<cplx, Lh, TR, NT>), grid, dim3(NT), 0, st, buf, twN, twh);
+  else
+    hipLaunchKernelGGL((k_fft_rows_r2c<cplx, Lh, TR, NT>), grid, dim3(NT), 0, st, buf, twN, twh);
+  return hipGetLastError();
+}
+
+#define EZ_FFT_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) \
+  X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048)
+
+bool fft_supported(int Ng) {
+  switch (Ng) {
+#define X(n) case n:
+    EZ_FFT_SIZES(X)
+#undef X
+      return true;
+    default: return false;
+  }
+}
+
+hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
+    const cplx *tw, int Ng, int axis, int sign, int mode, int comp,
+    hipStream_t st) {
+  const int Ngh = Ng / 2;
+  switch (Ng) {
+#define X(n) case n: return (sign > 0) \
+      ? launch_strided<n, +1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st) \
+      : launch_strided<n, -1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+    EZ_FFT_SIZES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
+    int sign, hipStream_t st) {
+  const size_t nrows = (size_t) Ng * Ng;
+  switch (Ng / 2) {
+#define X(n) case n / 2: return launch_rows<n / 2>(buf, twN, twh, nrows, sign, st);
+    EZ_FFT_SIZES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}

@@ -1,0 +1,341 @@
+// field.hip -- initial-condition spectrum and CIC density on gfx950.
+//
+//   k_pk_table    P(k) at every integer |n|^2 of the grid (pk_interp of
+//                 linear_pk.c:194-202 evaluated once per distinct |k|).
+//   k_gen_rhok    noise + Zel'dovich spectrum, perturb.c:78-359.
+//   k_cic         CIC mass assignment, dens_field.c:233-310, and the
+//                 momentum painting of pop_tracer.c:593-675 (3 components
+//                 in one pass over the particles).
+// This file is compiled with -ffp-contract=off: the particle coordinates and
+// CIC weights are formed with the same IEEE operations as the reference.
+#include "common.h"
+#include "rng.h"
+#include "kernels.h"
+
+// ------------------------------------------------------------ P(k) table
+// linear_pk.c:36-51 (bin search) and math/cspline.c:91-105 (evaluation).
+__device__ __forceinline__ double pk_eval(const double *__restrict__ kt,
+    const double *__restrict__ Pt, const double *__restrict__ P2t, int n,
+    double v) {
+  int l = 0, u = n - 1, idx = -1;
+  while (l <= u) {
+    const int i = (l + u) >> 1;
+    if (i >= n - 1) { idx = (kt[n - 1] == v) ? n - 1 : -1; break; }
+    if (kt[i + 1] <= v) l = i + 1;
+    else if (kt[i] > v) u = i - 1;
+    else { idx = i; break; }
+  }
+  if (idx == n - 1) return Pt[idx];
+  if (idx < 0) {
+    if (v >= kt[n - 1]) return Pt[n - 1];
+    return Pt[0];
+  }
+  const int j = idx + 1;
+  const double h = kt[j] - kt[idx];
+  const double dl = v - kt[idx];
+  const double dr = kt[j] - v;
+  const double h2 = h * h;
+  const double lin = dl * Pt[j] + dr * Pt[idx];
+  const double cr = (dl * dl - h2) * dl * P2t[j];
+  const double cl = (dr * dr - h2) * dr * P2t[idx];
+  return (lin + 0x1.5555555555555p-3 * (cr + cl)) / h;
+}
+
+__global__ void k_pk_table(double *__restrict__ tab, int nsq,
+    const double *__restrict__ kt, const double *__restrict__ Pt,
+    const double *__restrict__ P2t, int n, int logpk, double kfac,
+    double logkfac) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nsq) return;
+  if (q == 0) { tab[0] = 0; return; }
+  const double ksq = (double) q;
+  const double kmod = logpk ? log(ksq) * 0.5 + logkfac : kfac * sqrt(ksq);
+  tab[q] = pk_eval(kt, Pt, P2t, n, kmod);
+}
+
+// ------------------------------------------------------------ spectrum
+// One thread = CH consecutive cells of the half-complex grid (reference
+// order, k fastest, Ngk = Ng/2 + 1 per row).  Cell idx >= 1 consumes draws
+// (idx-1)*2 (amplitude, then phase) or (idx-1) with fixed amplitudes.
+// Output: rhok[i][j][kk < Ng/2] and rhok_ny[i][j] (the kz = Ng/2 plane);
+// optional copy in the reference layout.
+template <int CH>
+__global__ void __launch_bounds__(256)
+k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
+    real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
+    const uint32_t *__restrict__ jtab) {
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t idx0 = t * CH;
+  if (idx0 >= gp.Nh) return;
+  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
+  const size_t plane = (size_t) Ng * Ngk;
+  MrgState g = gp.seed_state;
+  const uint64_t mult = gp.fixamp ? 1 : 2;
+  mrg_jump(g, (idx0 == 0 ? 0 : (idx0 - 1)) * mult, jtab);
+  for (int c = 0; c < CH; c++) {
+    const size_t idx = idx0 + c;
+    if (idx >= gp.Nh) break;
+    const int i = (int) (idx / plane);
+    const size_t rem = idx - (size_t) i * plane;
+    const int j = (int) (rem / Ngk);
+    const int k = (int) (rem - (size_t) j * Ngk);
+    real re = 0, im = 0;
+    bool write = true, mirror = false;
+    int ni = 0, nj = 0;
+    if (idx != 0) {
+      const double amp = gp.fixamp ? 1.0 : mrg_next(g);
+      double ph = mrg_next(g) * 2 * EZ_PI;
+      const bool plane_k = (k == 0) || (k == Ngh && (Ng & 1) == 0);
+      if (plane_k) {
+        ni = (i == 0) ? 0 : Ng - i;
+        nj = (j == 0) ? 0 : Ng - j;
+        const bool lower = (i == ni && j <= Ngh) || (i != ni && i <= Ngh);
+        if (!lower) write = false;      // filled by its mirror
+        mirror = lower;
+      }
+      if (write) {
+        const int ki = (i <= Ngh) ? i : i - Ng;
+        const int kj = (j <= Ngh) ? j : j - Ng;
+        const int ksqi = ki * ki + kj * kj + k * k;
+        const double ksq = (double) ksqi;
+        double P = ptab[ksqi];
+        double a = gp.fixamp ? 1.0 : -log(amp);
+        if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
+        else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
+        P *= gp.Pfac;
+        if (gp.iphase) ph += EZ_PI;
+        P /= (ksq * gp.kfac);
+        re = (real) (-P * sin(ph));
+        im = (real) (P * cos(ph));
+        // perturb.c:341-359: self-conjugate modes keep only the imaginary
+        // part (the reference writes (+P sin, P cos) twice, then zeroes re).
+        if (mirror && i == ni && j == nj) re = 0;
+      }
+    }
+    else {
+      write = true;                     // (0,0,0) = 0
+    }
+    if (write) {
+      const cplx v = {re, im};
+      if (k < Ngh || (Ng & 1)) rhok[((size_t) i * Ng + j) * Ngh + k] = v;
+      else rhok_ny[(size_t) i * Ng + j] = v;
+      if (ref_copy) { ref_copy[2 * idx] = re; ref_copy[2 * idx + 1] = im; }
+      if (mirror && !(i == ni && j == nj)) {
+        const cplx w = {-re, im};
+        if (k < Ngh) rhok[((size_t) ni * Ng + nj) * Ngh + k] = w;
+        else rhok_ny[(size_t) ni * Ng + nj] = w;
+        if (ref_copy) {
+          const size_t nidx = ((size_t) ni * Ng + nj) * Ngk + k;
+          ref_copy[2 * nidx] = -re; ref_copy[2 * nidx + 1] = im;
+        }
+      }
+    }
+  }
+}
+
+hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
+    const GenParams &gp, const double *ptab, const uint32_t *jtab,
+    hipStream_t st) {
+  constexpr int CH = 16;
+  const size_t nthr = (gp.Nh + CH - 1) / CH;
+  hipLaunchKernelGGL(k_gen_rhok<CH>, dim3(ez_blocks(nthr, 256)), dim3(256), 0,
+      st, rhok, rhok_ny, ref_copy, gp, ptab, jtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
+    const double *Pt, const double *P2t, int n, int logpk, double kfac,
+    double logkfac, hipStream_t st) {
+  hipLaunchKernelGGL(k_pk_table, dim3(ez_blocks(nsq, 256)), dim3(256), 0, st,
+      tab, nsq, kt, Pt, P2t, n, logpk, kfac, logkfac);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ CIC
+__device__ __forceinline__ void atomic_add_real(real *p, real v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// One thread per particle (Lagrangian cell), k fastest.  MOM = 0: density
+// (dens_field.c:260-306); MOM = 1: the three momentum fields psi_a * w
+// (pop_tracer.c:626-671, weights scaled through wx0/wx1 as there).
+template <int MOM>
+__global__ void __launch_bounds__(256)
+k_cic(const real *__restrict__ p0, const real *__restrict__ p1,
+    const real *__restrict__ p2, real *__restrict__ o0, real *__restrict__ o1,
+    real *__restrict__ o2, int Ng, real igs, size_t N) {
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const int k = (int) (idx % Ng);
+  const size_t ij = idx / Ng;
+  const int j = (int) (ij % Ng);
+  const int i = (int) (ij / Ng);
+  const real q0 = p0[idx], q1 = p1[idx], q2 = p2[idx];
+  const real x = q0 * igs + (real) i;
+  const real y = q1 * igs + (real) j;
+  const real z = q2 * igs + (real) k;
+  int x0 = (int) floor((double) x), y0 = (int) floor((double) y),
+      z0 = (int) floor((double) z);
+  const real wx1 = x - (real) x0, wx0 = 1 - wx1;
+  const real wy1 = y - (real) y0, wy0 = 1 - wy1;
+  const real wz1 = z - (real) z0, wz0 = 1 - wz1;
+  if (x0 < 0) x0 += Ng; else if (x0 >= Ng) x0 -= Ng;
+  if (y0 < 0) y0 += Ng; else if (y0 >= Ng) y0 -= Ng;
+  if (z0 < 0) z0 += Ng; else if (z0 >= Ng) z0 -= Ng;
+  // Displacements beyond one box length (or non-finite ones) are undefined
+  // behaviour in the reference; keep every index inside the grid.
+  if ((unsigned) x0 >= (unsigned) Ng) x0 = ((x0 % Ng) + Ng) % Ng;
+  if ((unsigned) y0 >= (unsigned) Ng) y0 = ((y0 % Ng) + Ng) % Ng;
+  if ((unsigned) z0 >= (unsigned) Ng) z0 = ((z0 % Ng) + Ng) % Ng;
+  const int x1 = (x0 == Ng - 1) ? 0 : x0 + 1;
+  const int y1 = (y0 == Ng - 1) ? 0 : y0 + 1;
+  const int z1 = (z0 == Ng - 1) ? 0 : z0 + 1;
+  const size_t a00 = ((size_t) x0 * Ng + y0) * Ng, a01 = ((size_t) x0 * Ng + y1) * Ng;
+  const size_t a10 = ((size_t) x1 * Ng + y0) * Ng, a11 = ((size_t) x1 * Ng + y1) * Ng;
+  if (MOM == 0) {
+    atomic_add_real(o0 + a00 + z0, wx0 * wy0 * wz0);
+    atomic_add_real(o0 + a00 + z1, wx0 * wy0 * wz1);
+    atomic_add_real(o0 + a01 + z0, wx0 * wy1 * wz0);
+    atomic_add_real(o0 + a01 + z1, wx0 * wy1 * wz1);
+    atomic_add_real(o0 + a10 + z0, wx1 * wy0 * wz0);
+    atomic_add_real(o0 + a10 + z1, wx1 * wy0 * wz1);
+    atomic_add_real(o0 + a11 + z0, wx1 * wy1 * wz0);
+    atomic_add_real(o0 + a11 + z1, wx1 * wy1 * wz1);
+  }
+  else {
+    const real qs[3] = {q0, q1, q2};
+    real *os[3] = {o0, o1, o2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const real u0 = wx0 * qs[d], u1 = wx1 * qs[d];
+      real *o = os[d];
+      atomic_add_real(o + a00 + z0, u0 * wy0 * wz0);
+      atomic_add_real(o + a00 + z1, u0 * wy0 * wz1);
+      atomic_add_real(o + a01 + z0, u0 * wy1 * wz0);
+      atomic_add_real(o + a01 + z1, u0 * wy1 * wz1);
+      atomic_add_real(o + a10 + z0, u1 * wy0 * wz0);
+      atomic_add_real(o + a10 + z1, u1 * wy0 * wz1);
+      atomic_add_real(o + a11 + z0, u1 * wy1 * wz0);
+      atomic_add_real(o + a11 + z1, u1 * wy1 * wz1);
+    }
+  }
+}
+
+hipError_t launch_cic(const real *p0, const real *p1, const real *p2,
+    real *o0, real *o1, real *o2, int Ng, real igs, int momentum,
+    hipStream_t st) {
+  const size_t N = (size_t) Ng * Ng * Ng;
+  if (momentum)
+    hipLaunchKernelGGL(k_cic<1>, dim3(ez_blocks(N, 256)), dim3(256), 0, st,
+        p0, p1, p2, o0, o1, o2, Ng, igs, N);
+  else
+    hipLaunchKernelGGL(k_cic<0>, dim3(ez_blocks(N, 256)), dim3(256), 0, st,
+        p0, p1, p2, o0, o1, o2, Ng, igs, N);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------- utilities
+__global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, size_t n) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = (size_t) gridDim.x * blockDim.x;
+  for (size_t q = i; q < n; q += s) p[q] = v;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st) {
+  // bytes is a multiple of 4 for every buffer of the engine
+  const size_t n = bytes / 4;
+  unsigned nb = ez_blocks(n, 256);
+  if (nb > 65536) nb = 65536;
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, st, (uint32_t *) p, 0u, n);
+  return hipGetLastError();
+}
+
+// White-noise path, perturb.c:456-554: rhok = i sqrt(P) L^-1.5 / (k_f |n|^2)
+// * delta_k with delta_k from the r2c of the input field.  Input: the r2c
+// result in the packed layout (src[i][j][kk], kk = 0 slot = X0 + i XNy
+// after unpacking by k_unpack_r2c); output in the generator's layout.
+__global__ void k_gen_rhok_wn(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
+    const cplx *__restrict__ dk, const cplx *__restrict__ dk_ny, GenParams gp,
+    const double *__restrict__ ptab, real *__restrict__ ref_copy) {
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= gp.Nh) return;
+  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
+  const size_t plane = (size_t) Ng * Ngk;
+  const int i = (int) (idx / plane);
+  const size_t rem = idx - (size_t) i * plane;
+  const int j = (int) (rem / Ngk);
+  const int k = (int) (rem - (size_t) j * Ngk);
+  const size_t ij = (size_t) i * Ng + j;
+  cplx d = (k < Ngh) ? dk[ij * Ngh + k] : dk_ny[ij];
+  real re = 0, im = 0;
+  if (idx != 0) {
+    const int ki = (i <= Ngh) ? i : i - Ng;
+    const int kj = (j <= Ngh) ? j : j - Ng;
+    const int ksqi = ki * ki + kj * kj + k * k;
+    double P = ptab[ksqi];
+    P = gp.logpk ? exp(P * 0.5) : sqrt(P);
+    P *= gp.Pfac / ((double) ksqi * gp.kfac);
+    re = (real) (-P * d.y);
+    im = (real) (P * d.x);
+  }
+  const cplx v = {re, im};
+  if (k < Ngh) rhok[ij * Ngh + k] = v;
+  else rhok_ny[ij] = v;
+  if (ref_copy) { ref_copy[2 * idx] = re; ref_copy[2 * idx + 1] = im; }
+}
+
+hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
+    const cplx *dk_ny, const GenParams &gp, const double *ptab,
+    real *ref_copy, hipStream_t st) {
+  hipLaunchKernelGGL(k_gen_rhok_wn, dim3(ez_blocks(gp.Nh, 256)), dim3(256), 0,
+      st, rhok, rhok_ny, dk, dk_ny, gp, ptab, ref_copy);
+  return hipGetLastError();
+}
+
+// After the forward (i, j) transforms of a packed r2c field, slot kk = 0
+// holds A + i B with A = FFT2(x0), B = FFT2(xNy) of two real planes.
+// A(k) = (P(k) + conj(P(-k)))/2, B(k) = (P(k) - conj(P(-k)))/(2i).
+__global__ void k_unpack_r2c(cplx *__restrict__ buf, cplx *__restrict__ ny,
+    int Ng) {
+  const size_t ij = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t plane = (size_t) Ng * Ng;
+  if (ij >= plane) return;
+  const int Ngh = Ng >> 1;
+  const int i = (int) (ij / Ng), j = (int) (ij % Ng);
+  const int ni = (i == 0) ? 0 : Ng - i, nj = (j == 0) ? 0 : Ng - j;
+  const cplx p = buf[ij * Ngh];
+  const cplx q = buf[((size_t) ni * Ng + nj) * Ngh];
+  const cplx a = {(real) 0.5 * (p.x + q.x), (real) 0.5 * (p.y - q.y)};
+  const cplx b = {(real) 0.5 * (p.y + q.y), (real) 0.5 * (q.x - p.x)};
+  ny[ij] = b;
+  // in-place write of A must not race with the read of the mirror: the
+  // caller passes a separate output plane for A through `ny + plane`.
+  ny[plane + ij] = a;
+}
+
+__global__ void k_store_slot0(cplx *__restrict__ buf, const cplx *__restrict__ a,
+    int Ng) {
+  const size_t ij = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (ij >= (size_t) Ng * Ng) return;
+  buf[ij * (Ng >> 1)] = a[ij];
+}
+
+hipError_t launch_unpack_r2c(cplx *buf, cplx *ny2, int Ng, hipStream_t st) {
+  const size_t plane = (size_t) Ng * Ng;
+  hipLaunchKernelGGL(k_unpack_r2c, dim3(ez_blocks(plane, 256)), dim3(256), 0,
+      st, buf, ny2, Ng);
+  hipLaunchKernelGGL(k_store_slot0, dim3(ez_blocks(plane, 256)), dim3(256), 0,
+      st, buf, ny2 + plane, Ng);
+  return hipGetLastError();
+}
+
+__global__ void k_scale(real *__restrict__ p, real s, size_t n) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] *= s;
+}
+
+hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(ez_blocks(n, 256)), dim3(256), 0, st, p, s, n);
+  return hipGetLastError();
+}

@@ -1,0 +1,72 @@
+// kernels.h -- launch wrappers of the gfx950 kernels (field.hip, tracer.hip).
+#pragma once
+#include "common.h"
+#include "rng.h"
+
+struct GenParams {
+  int Ng;
+  size_t Nh;                    // Ng*Ng*(Ng/2+1) half-complex cells
+  double kfac, logkfac, Pfac;
+  int logpk, fixamp, iphase;
+  MrgState seed_state;
+};
+
+// field.hip
+hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
+    const double *Pt, const double *P2t, int n, int logpk, double kfac,
+    double logkfac, hipStream_t st);
+hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
+    const GenParams &gp, const double *ptab, const uint32_t *jtab,
+    hipStream_t st);
+hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
+    const cplx *dk_ny, const GenParams &gp, const double *ptab,
+    real *ref_copy, hipStream_t st);
+hipError_t launch_unpack_r2c(cplx *buf, cplx *ny2, int Ng, hipStream_t st);
+hipError_t launch_cic(const real *p0, const real *p1, const real *p2,
+    real *o0, real *o1, real *o2, int Ng, real igs, int momentum,
+    hipStream_t st);
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st);
+
+// tracer.hip
+struct BiasParams {
+  size_t N;
+  real thres, rho_sat;
+  double iexp;
+  double lambda;
+  int imax;
+  MrgState seed_state;
+};
+
+// Cells per block of the per-cell passes (256 threads x 16 cells).
+#define EZ_CELLS_PER_BLOCK 4096
+
+hipError_t launch_count_valid(const real *rho, real thres, size_t N,
+    uint32_t *blockcnt, hipStream_t st);
+hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
+    uint64_t *tmp, hipStream_t st);
+size_t scan_tmp_words(size_t n);
+hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
+    unsigned long long *hist, const BiasParams &bp, const uint32_t *jtab,
+    hipStream_t st);
+hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
+    size_t cap, unsigned long long *counter, hipStream_t st);
+hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
+    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st);
+size_t sort_hist_words(size_t n);
+hipError_t launch_walk(const rkey *sorted, size_t M, const uint64_t *num,
+    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st);
+hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
+    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st);
+hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
+    int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
+    real *x, real *y, real *z, size_t ntot, hipStream_t st);
+hipError_t launch_velocities(const real *x, const real *y, const real *z,
+    const real *rho, const real *m0, const real *m1, const real *m2,
+    size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
+    hipStream_t st);
+hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
+    const uint32_t *jtab, double *out, hipStream_t st);
+hipError_t launch_rsd(const real *z, const real *vz, size_t n, double fac,
+    double Lbox, double *out, hipStream_t st);

@@ -1,0 +1,677 @@
+// tracer.hip -- effective-bias tracer population on gfx950.
+//
+// Stages of EZmock_populate_tracer (pop_tracer.c:1140-1219):
+//   k_count_valid + scan + k_bias   bias_model, pop_tracer.c:149-348: valid
+//                                   cell of rank r uses the Box-Muller pair
+//                                   r/2 drawn at 2N + 2(r/2).
+//   k_compact + radix sort + k_walk n_tracer_in_cell, :359-514: the PDF
+//                                   bins are rank ranges of the sorted tracer
+//                                   densities, each extended over the ties at
+//                                   its lowest value (:436-446).
+//   k_assign_nt + scan              tracers per cell and their offsets.
+//   k_positions                     generate_tracers, :881-1007: tracer t at
+//                                   draws 5N + 3t.
+//   k_velocities                    disp_cic normalisation (:676-682) fused
+//                                   into assign_vel (:697-753) and
+//                                   add_vel_scatter (:1043-1113).
+// Compiled with -ffp-contract=off (IEEE operation order of the reference).
+#include "common.h"
+#include "rng.h"
+#include "kernels.h"
+
+#define CPB EZ_CELLS_PER_BLOCK
+#define TPB 256
+#define CPT (CPB / TPB)
+
+// ------------------------------------------------------------ block scan
+template <class T>
+__device__ __forceinline__ T block_excl_scan(T v, T &total, T *ws) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  T off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < TPB / 64; i++) {
+    T s = ws[i];
+    if (i < w) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+// ------------------------------------------------------------ u32 scans
+__global__ void __launch_bounds__(TPB)
+k_scan_chunks(const uint32_t *__restrict__ in, uint64_t *__restrict__ out,
+    size_t n, uint64_t *__restrict__ sums) {
+  __shared__ uint64_t ws[TPB / 64];
+  const size_t base = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t v[CPT];
+  uint64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    v[q] = (base + q < n) ? in[base + q] : 0u;
+    s += v[q];
+  }
+  uint64_t tot;
+  uint64_t off = block_excl_scan<uint64_t>(s, tot, ws);
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    if (base + q < n) out[base + q] = off;
+    off += v[q];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// In-place exclusive scan of up to CPB u64 values by one block.
+__global__ void __launch_bounds__(TPB)
+k_scan_top(uint64_t *__restrict__ s, size_t n, uint64_t *__restrict__ total) {
+  __shared__ uint64_t ws[TPB / 64];
+  const size_t base = (size_t) threadIdx.x * CPT;
+  uint64_t v[CPT], t = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    v[q] = (base + q < n) ? s[base + q] : 0;
+    t += v[q];
+  }
+  uint64_t tot;
+  uint64_t off = block_excl_scan<uint64_t>(t, tot, ws);
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    if (base + q < n) s[base + q] = off;
+    off += v[q];
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+__global__ void k_scan_add(uint64_t *__restrict__ out, size_t n,
+    const uint64_t *__restrict__ sums) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / CPB];
+}
+
+size_t scan_tmp_words(size_t n) { return (n + CPB - 1) / CPB + 1; }
+
+// Exclusive scan of n <= CPB*CPB u32 into u64; out[n] is not written, the
+// grand total lands in tmp[nchunks].
+hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
+    uint64_t *tmp, hipStream_t st) {
+  const size_t nch = (n + CPB - 1) / CPB;
+  if (nch > CPB) return hipErrorInvalidValue;
+  if (nch == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned) nch), dim3(TPB), 0, st, in, out, n, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TPB), 0, st, tmp, nch, tmp + nch);
+  hipLaunchKernelGGL(k_scan_add, dim3(ez_blocks(n, 256)), dim3(256), 0, st, out, n, (const uint64_t *) tmp);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ bias model
+__global__ void __launch_bounds__(TPB)
+k_count_valid(const real *__restrict__ rho, real thres, size_t N,
+    uint32_t *__restrict__ blockcnt) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t base = (size_t) blockIdx.x * CPB;
+  uint32_t c = 0;
+#pragma unroll 4
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = base + (size_t) q * TPB + threadIdx.x;
+    if (i < N && rho[i] >= thres) c++;
+  }
+  uint32_t tot;
+  (void) block_excl_scan<uint32_t>(c, tot, ws);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+hipError_t launch_count_valid(const real *rho, real thres, size_t N,
+    uint32_t *blockcnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_count_valid, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
+      rho, thres, N, blockcnt);
+  return hipGetLastError();
+}
+
+// Histogram bin of a tracer density, pop_tracer.c:265-268: imax - ilogb(v)
+// clamped to [0, 63], the subtraction wrapping as 32-bit int arithmetic.
+__device__ __forceinline__ int hist_bin(real v, int imax) {
+  int d = (int) ((unsigned) imax - (unsigned) ilogb(v));
+  if (d < 0) d = 0;
+  else if (d >= EZ_NUM_LOGB_BIN) d = EZ_NUM_LOGB_BIN - 1;
+  return d;
+}
+
+// pop_tracer.c:70-79
+__device__ __forceinline__ void gauss_pair(MrgState &g, double sigma,
+    double &g1, double &g2) {
+  const double r = -2 * log(mrg_next(g));
+  const double th = mrg_next(g) * 2 * EZ_PI;
+  g1 = sigma * sqrt(r) * cos(th);
+  g2 = sigma * sqrt(r) * sin(th);
+}
+
+// LDS tile with one pad word per 16 so that thread-sequential access
+// (thread t reads cells 16t .. 16t+15) is bank-conflict free.
+#define PADIDX(c) ((c) + ((c) >> 4))
+
+__global__ void __launch_bounds__(TPB)
+k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
+    const uint64_t *__restrict__ blockoff, unsigned long long *__restrict__ hist,
+    BiasParams bp, const uint32_t *__restrict__ jtab) {
+  __shared__ real tile[PADIDX(CPB)];
+  __shared__ uint32_t lh[EZ_NUM_LOGB_BIN];
+  __shared__ uint64_t ws[TPB / 64];
+  __shared__ MrgState base;
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  const int tid = threadIdx.x;
+  if (tid < EZ_NUM_LOGB_BIN) lh[tid] = 0;
+#pragma unroll 4
+  for (int q = 0; q < CPT; q++) {
+    const int c = q * TPB + tid;
+    const size_t i = b0 + c;
+    tile[PADIDX(c)] = (i < bp.N) ? rho[i] : (real) 0;
+  }
+  __syncthreads();
+  // thread tid owns cells c = 16 tid .. 16 tid + 15 of the block
+  uint64_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int c = tid * CPT + q;
+    if (b0 + c < bp.N && tile[PADIDX(c)] >= bp.thres) cnt++;
+  }
+  uint64_t tot;
+  const uint64_t excl = block_excl_scan<uint64_t>(cnt, tot, ws);
+  const uint64_t rb = blockoff[blockIdx.x];
+  const uint64_t pb = rb >> 1;
+  if (tid == 0) {
+    MrgState g = bp.seed_state;
+    mrg_jump(g, (uint64_t) 2 * bp.N + 2 * pb, jtab);
+    base = g;
+  }
+  __syncthreads();
+  uint64_t r = rb + excl;
+  MrgState g = base;
+  if (cnt) mrg_jump(g, 2 * ((r >> 1) - pb), jtab);
+  double gg[2] = {0, 0};
+  bool have = false;
+#pragma unroll 1
+  for (int q = 0; q < CPT; q++) {
+    const int c = tid * CPT + q;
+    if (b0 + c >= bp.N) break;
+    const real v = tile[PADIDX(c)];
+    if (v < bp.thres) { tile[PADIDX(c)] = 0; continue; }
+    const real rt = (v > bp.rho_sat) ? bp.rho_sat : v;
+    if ((r & 1) == 0 || !have) {
+      gauss_pair(g, bp.lambda, gg[0], gg[1]);
+      gg[0] = (gg[0] >= 0) ? 1 + gg[0] : exp(gg[0]);
+      gg[1] = (gg[1] >= 0) ? 1 + gg[1] : exp(gg[1]);
+      have = true;
+    }
+    const double tmp = (1 - exp((double) rt * bp.iexp)) * gg[r & 1];
+    const real o = (real) tmp;
+    tile[PADIDX(c)] = o;
+    atomicAdd(&lh[hist_bin(o, bp.imax)], 1u);
+    r++;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int q = 0; q < CPT; q++) {
+    const int c = q * TPB + tid;
+    const size_t i = b0 + c;
+    if (i < bp.N) rhot[i] = tile[PADIDX(c)];
+  }
+  if (tid < EZ_NUM_LOGB_BIN && lh[tid])
+    atomicAdd(&hist[tid], (unsigned long long) lh[tid]);
+}
+
+hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
+    unsigned long long *hist, const BiasParams &bp, const uint32_t *jtab,
+    hipStream_t st) {
+  hipLaunchKernelGGL(k_bias, dim3(ez_blocks(bp.N, CPB)), dim3(TPB), 0, st,
+      rho, rhot, blockoff, hist, bp, jtab);
+  return hipGetLastError();
+}
+
+// -------------------------------------------------------------- compact
+// Keys: bitwise NOT of the bits of a positive real, so that ascending key
+// order is descending density.
+__device__ __forceinline__ rkey real_key(real v) {
+#ifdef SINGLE_PREC
+  return ~__float_as_uint(v);
+#else
+  return ~(uint64_t) __double_as_longlong(v);
+#endif
+}
+
+__global__ void __launch_bounds__(TPB)
+k_compact(const real *__restrict__ rhot, real thres, size_t N,
+    rkey *__restrict__ keys, size_t cap, unsigned long long *__restrict__ counter) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const bool take = (i < N) && rhot[i] >= thres;
+  const uint64_t m = __ballot(take);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long) m) - 1;
+  unsigned long long b = 0;
+  if (lane == leader) b = atomicAdd(counter, (unsigned long long) __popcll(m));
+  b = __shfl(b, leader, 64);
+  if (take) {
+    const uint64_t below = m & ((1ull << lane) - 1);
+    const size_t pos = b + __popcll(below);
+    if (pos < cap) keys[pos] = real_key(rhot[i]);
+  }
+}
+
+hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
+    size_t cap, unsigned long long *counter, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact, dim3(ez_blocks(N, TPB)), dim3(TPB), 0, st,
+      rhot, thres, N, keys, cap, counter);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------- radix sort
+// LSD, 8-bit digits, 4096 keys per block; stable ranks within a block from
+// wave ballots (lanes with equal digit) and per-wave digit counts.
+#define SORT_ITEMS CPT
+
+__global__ void __launch_bounds__(TPB)
+k_sort_hist(const rkey *__restrict__ in, size_t n, int shift,
+    uint32_t *__restrict__ gh, size_t nblk) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t) blockIdx.x * CPB;
+#pragma unroll 4
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const size_t e = base + (size_t) it * TPB + threadIdx.x;
+    if (e < n) atomicAdd(&h[(uint32_t) (in[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  gh[(size_t) threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(TPB)
+k_sort_scatter(const rkey *__restrict__ in, rkey *__restrict__ out, size_t n,
+    int shift, const uint64_t *__restrict__ gofs, size_t nblk) {
+  __shared__ uint64_t bofs[256];
+  __shared__ uint32_t wcnt[TPB / 64][256];
+  __shared__ uint32_t wofs[TPB / 64][256];
+  __shared__ uint32_t run[256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  bofs[tid] = gofs[(size_t) tid * nblk + blockIdx.x];
+  run[tid] = 0;
+#pragma unroll
+  for (int q = 0; q < TPB / 64; q++) wcnt[q][tid] = 0;
+  __syncthreads();
+  const size_t base = (size_t) blockIdx.x * CPB;
+  const uint64_t lt = (1ull << lane) - 1;
+#pragma unroll 1
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const size_t e = base + (size_t) it * TPB + tid;
+    const bool valid = e < n;
+    const rkey key = valid ? in[e] : (rkey) 0;
+    const uint32_t d = (uint32_t) (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t bb = __ballot(valid && on);
+      peers &= on ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t) __popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t) __popcll(peers);
+    __syncthreads();
+    {
+      uint32_t r = run[tid];
+#pragma unroll
+      for (int q = 0; q < TPB / 64; q++) {
+        const uint32_t c = wcnt[q][tid];
+        wofs[q][tid] = r;
+        r += c;
+        wcnt[q][tid] = 0;
+      }
+      run[tid] = r;
+    }
+    __syncthreads();
+    if (valid) out[bofs[d] + wofs[w][d] + rank] = key;
+  }
+}
+
+size_t sort_hist_words(size_t n) { return 256 * ((n + CPB - 1) / CPB); }
+
+// Sorts keys[0..n) ascending; tmp has n entries; ghist holds
+// sort_hist_words(n) u32; gofs the same number of u64; scan_tmp
+// scan_tmp_words(sort_hist_words(n)) u64.
+hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
+    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const size_t nblk = (n + CPB - 1) / CPB;
+  rkey *src = keys, *dst = tmp;
+  const int npass = (int) sizeof(rkey);
+  for (int p = 0; p < npass; p++) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned) nblk), dim3(TPB), 0, st,
+        src, n, shift, ghist, nblk);
+    hipError_t e = launch_scan_u32(ghist, gofs, 256 * nblk, scan_tmp, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned) nblk), dim3(TPB), 0, st,
+        src, dst, n, shift, gofs, nblk);
+    rkey *t = src; src = dst; dst = t;
+  }
+  // npass is even (4 or 8): the result is back in `keys`.
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- PDF walk
+// One wave.  sorted: ascending keys (descending density), M of them.
+// For PDF bin p (nmax - p tracers per cell): its last rank is s + num[p] - 1
+// and the bin takes every further key equal to the key there.
+// Outputs: vb[p] = largest key of bin p (0 when bins 0..p are empty),
+// cnt[p] = cells in bin p, res = {ncell, ntot, error}.
+__global__ void __launch_bounds__(64)
+k_walk(const rkey *__restrict__ sorted, size_t M, const uint64_t *__restrict__ num,
+    int nmax, rkey *__restrict__ vb, uint64_t *__restrict__ cnt,
+    uint64_t *__restrict__ res) {
+  const int lane = threadIdx.x;
+  size_t s = 0;
+  uint64_t ntot = 0;
+  int err = 0;
+  rkey last = 0;
+  for (int p = 0; p < nmax; p++) {
+    const uint64_t n = num[p];
+    if (n == 0) {
+      if (lane == 0) { vb[p] = last; cnt[p] = 0; }
+      continue;
+    }
+    const size_t e = s + n;
+    if (e > M) { err = 1; break; }
+    const rkey v = sorted[e - 1];
+    size_t pos = e;
+    for (;;) {
+      const size_t q = pos + lane;
+      const bool stop = (q >= M) || (sorted[q] != v);
+      const uint64_t m = __ballot(stop);
+      if (m) { pos += __ffsll((unsigned long long) m) - 1; break; }
+      pos += 64;
+    }
+    if (lane == 0) { vb[p] = v; cnt[p] = pos - s; }
+    ntot += (uint64_t) (pos - s) * (uint64_t) (nmax - p);
+    last = v;
+    s = pos;
+  }
+  if (lane == 0) { res[0] = s; res[1] = ntot; res[2] = err; }
+}
+
+hipError_t launch_walk(const rkey *sorted, size_t M, const uint64_t *num,
+    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st) {
+  hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, st, sorted, M, num, nmax,
+      vb, cnt, res);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------ tracers per cell
+__global__ void __launch_bounds__(TPB)
+k_assign_nt(const real *__restrict__ rhot, size_t N, const rkey *__restrict__ vb,
+    int nmax, int16_t *__restrict__ nt, uint32_t *__restrict__ blocktr) {
+  __shared__ rkey sb[EZ_MAX_NUM_TRACER_PER_CELL];
+  __shared__ uint32_t ws[TPB / 64];
+  for (int p = threadIdx.x; p < nmax; p += TPB) sb[p] = vb[p];
+  __syncthreads();
+  const rkey vlast = sb[nmax - 1];
+  const size_t base = (size_t) blockIdx.x * CPB;
+  uint32_t sum = 0;
+#pragma unroll 4
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = base + (size_t) q * TPB + threadIdx.x;
+    if (i >= N) break;
+    const real v = rhot[i];
+    int16_t c = 0;
+    if (v > 0) {
+      const rkey u = real_key(v);
+      if (u <= vlast) {
+        int lo = 0, hi = nmax - 1;     // first p with u <= vb[p]
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (u <= sb[mid]) hi = mid; else lo = mid + 1;
+        }
+        c = (int16_t) (nmax - lo);
+      }
+    }
+    nt[i] = c;
+    sum += (uint32_t) c;
+  }
+  uint32_t tot;
+  (void) block_excl_scan<uint32_t>(sum, tot, ws);
+  if (threadIdx.x == 0) blocktr[blockIdx.x] = tot;
+}
+
+hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
+    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st) {
+  hipLaunchKernelGGL(k_assign_nt, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
+      rhot, N, vb, nmax, nt, blocktr);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ positions
+__global__ void __launch_bounds__(TPB)
+k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockofs,
+    int Ng, double Lbox, real gs, size_t N, MrgState seed,
+    const uint32_t *__restrict__ jtab, real *__restrict__ x,
+    real *__restrict__ y, real *__restrict__ z, size_t ntot) {
+  __shared__ int16_t tile[PADIDX(CPB)];
+  __shared__ uint64_t ws[TPB / 64];
+  __shared__ MrgState base;
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int q = 0; q < CPT; q++) {
+    const int c = q * TPB + tid;
+    const size_t i = b0 + c;
+    tile[PADIDX(c)] = (i < N) ? nt[i] : (int16_t) 0;
+  }
+  __syncthreads();
+  uint64_t cnt = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) cnt += (uint64_t) tile[PADIDX(tid * CPT + q)];
+  uint64_t tot;
+  const uint64_t excl = block_excl_scan<uint64_t>(cnt, tot, ws);
+  if (tot == 0) return;
+  const uint64_t tb = blockofs[blockIdx.x];
+  if (tid == 0) {
+    MrgState g = seed;
+    mrg_jump(g, (uint64_t) 5 * N + 3 * tb, jtab);
+    base = g;
+  }
+  __syncthreads();
+  if (cnt == 0) return;
+  MrgState g = base;
+  mrg_jump(g, 3 * excl, jtab);
+  size_t t = tb + excl;
+#pragma unroll 1
+  for (int q = 0; q < CPT; q++) {
+    const int c = tid * CPT + q;
+    const int m = tile[PADIDX(c)];
+    if (m <= 0) continue;
+    const size_t cell = b0 + c;
+    const int k = (int) (cell % Ng);
+    const size_t ij = cell / Ng;
+    const int j = (int) (ij % Ng), i = (int) (ij / Ng);
+    for (int n = 0; n < m; n++) {
+      double r = mrg_next(g) - 0.5;
+      real xx = (real) ((i + r) * (double) gs);
+      r = mrg_next(g) - 0.5;
+      real yy = (real) ((j + r) * (double) gs);
+      r = mrg_next(g) - 0.5;
+      real zz = (real) ((k + r) * (double) gs);
+      if (xx < 0) xx = (real) ((double) xx + Lbox);
+      if (yy < 0) yy = (real) ((double) yy + Lbox);
+      if (zz < 0) zz = (real) ((double) zz + Lbox);
+      if (t < ntot) { x[t] = xx; y[t] = yy; z[t] = zz; }
+      t++;
+    }
+  }
+}
+
+hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
+    int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
+    real *x, real *y, real *z, size_t ntot, hipStream_t st) {
+  const size_t N = (size_t) Ng * Ng * Ng;
+  hipLaunchKernelGGL(k_positions, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
+      nt, blockofs, Ng, Lbox, gs, N, seed, jtab, x, y, z, ntot);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------- velocities
+// One thread = PPT tracer pairs.  For each tracer: CIC gather of the
+// momentum fields normalised by rho (disp_cic, pop_tracer.c:680-681, where
+// rho > REAL_TOL), weighted by rho (assign_vel), times vfac; then the
+// Gaussian scatter of pair q from draws 8N + 6q.
+#define PPT 4
+
+__device__ __forceinline__ void gather_vel(const real *__restrict__ rho,
+    const real *__restrict__ m0, const real *__restrict__ m1,
+    const real *__restrict__ m2, int Ng, real igs, double vfac, real px,
+    real py, real pz, real *v) {
+  const real xx = px * igs, yy = py * igs, zz = pz * igs;
+  int x0 = (int) xx, y0 = (int) yy, z0 = (int) zz;
+  const real wx1 = xx - (real) x0, wx0 = 1 - wx1;
+  const real wy1 = yy - (real) y0, wy0 = 1 - wy1;
+  const real wz1 = zz - (real) z0, wz0 = 1 - wz1;
+  if (x0 == Ng) x0 = 0;
+  if (y0 == Ng) y0 = 0;
+  if (z0 == Ng) z0 = 0;
+  if ((unsigned) x0 >= (unsigned) Ng) x0 = 0;    // only for non-finite input
+  if ((unsigned) y0 >= (unsigned) Ng) y0 = 0;
+  if ((unsigned) z0 >= (unsigned) Ng) z0 = 0;
+  const int x1 = (x0 == Ng - 1) ? 0 : x0 + 1;
+  const int y1 = (y0 == Ng - 1) ? 0 : y0 + 1;
+  const int z1 = (z0 == Ng - 1) ? 0 : z0 + 1;
+  const size_t c[8] = {
+    ((size_t) x0 * Ng + y0) * Ng + z0, ((size_t) x0 * Ng + y0) * Ng + z1,
+    ((size_t) x0 * Ng + y1) * Ng + z0, ((size_t) x0 * Ng + y1) * Ng + z1,
+    ((size_t) x1 * Ng + y0) * Ng + z0, ((size_t) x1 * Ng + y0) * Ng + z1,
+    ((size_t) x1 * Ng + y1) * Ng + z0, ((size_t) x1 * Ng + y1) * Ng + z1};
+  real rc[8], r[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) rc[q] = rho[c[q]];
+  // r_q = rho * wx * wy * wz, evaluated left to right as assign_vel does
+  r[0] = rc[0] * wx0 * wy0 * wz0; r[1] = rc[1] * wx0 * wy0 * wz1;
+  r[2] = rc[2] * wx0 * wy1 * wz0; r[3] = rc[3] * wx0 * wy1 * wz1;
+  r[4] = rc[4] * wx1 * wy0 * wz0; r[5] = rc[5] * wx1 * wy0 * wz1;
+  r[6] = rc[6] * wx1 * wy1 * wz0; r[7] = rc[7] * wx1 * wy1 * wz1;
+  const real rs = r[0] + r[1] + r[2] + r[3] + r[4] + r[5] + r[6] + r[7];
+  const real *ms[3] = {m0, m1, m2};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    real a[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const real mv = ms[d][c[q]];
+      a[q] = ((double) rc[q] > EZ_REAL_TOL) ? mv / rc[q] : mv;
+    }
+    real s = a[0] * r[0] + a[1] * r[1] + a[2] * r[2] + a[3] * r[3]
+           + a[4] * r[4] + a[5] * r[5] + a[6] * r[6] + a[7] * r[7];
+    v[d] = (real) ((double) s * (vfac / (double) rs));
+  }
+}
+
+__global__ void __launch_bounds__(TPB)
+k_velocities(const real *__restrict__ x, const real *__restrict__ y,
+    const real *__restrict__ z, const real *__restrict__ rho,
+    const real *__restrict__ m0, const real *__restrict__ m1,
+    const real *__restrict__ m2, size_t ntot, int Ng, real igs, double vfac,
+    double sigma, uint64_t base_draw, MrgState seed,
+    const uint32_t *__restrict__ jtab, real *__restrict__ vx,
+    real *__restrict__ vy, real *__restrict__ vz) {
+  __shared__ MrgState base;
+  const size_t npair = (ntot + 1) / 2;
+  const size_t q0blk = (size_t) blockIdx.x * TPB * PPT;
+  if (threadIdx.x == 0) {
+    MrgState g = seed;
+    mrg_jump(g, base_draw + 6 * q0blk, jtab);
+    base = g;
+  }
+  __syncthreads();
+  const size_t q0 = q0blk + (size_t) threadIdx.x * PPT;
+  if (q0 >= npair) return;
+  MrgState g = base;
+  mrg_jump(g, 6 * (q0 - q0blk), jtab);
+#pragma unroll 1
+  for (int p = 0; p < PPT; p++) {
+    const size_t q = q0 + p;
+    if (q >= npair) break;
+    const size_t a = 2 * q, b = a + 1;
+    real va[3], vb3[3];
+    gather_vel(rho, m0, m1, m2, Ng, igs, vfac, x[a], y[a], z[a], va);
+    double g1, g2;
+    gauss_pair(g, sigma, g1, g2);
+    va[0] = (real) ((double) va[0] + g1);
+    va[1] = (real) ((double) va[1] + g2);
+    gauss_pair(g, sigma, g1, g2);
+    va[2] = (real) ((double) va[2] + g1);
+    vx[a] = va[0]; vy[a] = va[1]; vz[a] = va[2];
+    if (b < ntot) {                       // else: the last, unpaired tracer
+      gather_vel(rho, m0, m1, m2, Ng, igs, vfac, x[b], y[b], z[b], vb3);
+      vb3[0] = (real) ((double) vb3[0] + g2);
+      gauss_pair(g, sigma, g1, g2);
+      vb3[1] = (real) ((double) vb3[1] + g1);
+      vb3[2] = (real) ((double) vb3[2] + g2);
+      vx[b] = vb3[0]; vy[b] = vb3[1]; vz[b] = vb3[2];
+    }
+  }
+}
+
+hipError_t launch_velocities(const real *x, const real *y, const real *z,
+    const real *rho, const real *m0, const real *m1, const real *m2,
+    size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
+    hipStream_t st) {
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const size_t npair = (ntot + 1) / 2;
+  if (npair == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_velocities, dim3(ez_blocks(npair, TPB * PPT)), dim3(TPB),
+      0, st, x, y, z, rho, m0, m1, m2, ntot, Ng, igs, vfac, sigma_v,
+      (uint64_t) 8 * N, seed, jtab, vx, vy, vz);
+  return hipGetLastError();
+}
+
+// -------------------------------------------------------- test helpers
+__global__ void k_uniform(MrgState seed, uint64_t start, size_t n,
+    const uint32_t *__restrict__ jtab, double *__restrict__ out) {
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t i0 = t * 64;
+  if (i0 >= n) return;
+  MrgState g = seed;
+  mrg_jump(g, start + i0, jtab);
+  for (size_t i = i0; i < i0 + 64 && i < n; i++) out[i] = mrg_next(g);
+}
+
+hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
+    const uint32_t *jtab, double *out, hipStream_t st) {
+  const size_t nthr = (n + 63) / 64;
+  hipLaunchKernelGGL(k_uniform, dim3(ez_blocks(nthr, 256)), dim3(256), 0, st,
+      seed, start, n, jtab, out);
+  return hipGetLastError();
+}
+
+// save_res.c:147-158 on the device.
+__global__ void k_rsd(const real *__restrict__ z, const real *__restrict__ vz,
+    size_t n, double fac, double Lbox, double *__restrict__ out) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double zr = (double) z[i] + fac * (double) vz[i];
+  if (zr >= Lbox) zr -= Lbox;
+  if (zr < 0) zr += Lbox;
+  out[i] = zr;
+}
+
+hipError_t launch_rsd(const real *z, const real *vz, size_t n, double fac,
+    double Lbox, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rsd, dim3(ez_blocks(n, 256)), dim3(256), 0, st, z, vz,
+      n, fac, Lbox, out);
+  return hipGetLastError();
+}

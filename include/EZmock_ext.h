@@ -1,0 +1,96 @@
+/*
+ * EZmock_ext.h -- extensions of the MI355X EZmock engine that have no
+ * counterpart in the reference API (kept out of EZmock.h so the drop-in ABI
+ * stays exactly the reference's).  Used by the benchmark, the parity tests and
+ * callers that keep the catalogue on the GPU.
+ */
+#ifndef EZMOCK_AMD_EZMOCK_EXT_H
+#define EZMOCK_AMD_EZMOCK_EXT_H
+
+#include "EZmock.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Build flavour: sizeof(real) in bytes (4 or 8). */
+int EZmock_ext_real_size(void);
+
+/* Number of visible HIP devices (0 without a GPU); never faults. */
+int EZmock_ext_device_count(void);
+
+/* Same as EZmock_populate_tracer but the catalogue stays in device memory
+   (read it with EZmock_ext_copy_tracers or EZmock_ext_device_tracers). */
+int EZmock_ext_populate_device(EZMOCK *ez, const real *params,
+    const size_t nexp, const bool att_part, size_t *ntracer, int *err);
+
+/* Copy the device-resident catalogue of the last populate call into six
+   caller-owned host arrays of at least `n` elements (n <= ntracer). */
+int EZmock_ext_copy_tracers(EZMOCK *ez, real *x, real *y, real *z, real *vx,
+    real *vy, real *vz, const size_t n, int *err);
+
+/* Device pointers of the catalogue (valid until the next populate/destroy). */
+int EZmock_ext_device_tracers(EZMOCK *ez, void *ptrs[6], size_t *ntracer,
+    int *err);
+
+/* Redshift-space z of the device-resident catalogue, computed on the GPU in
+   double as save_res.c:147-158 does: z + rsd_fac*vz wrapped once into
+   [0, Lbox).  Written to a caller-owned host array of `ntracer` doubles. */
+int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd,
+    int *err);
+
+/* Copy an internal field to host memory:
+     0,1,2  psi_x, psi_y, psi_z          Ng^3 real
+     3      rho (CIC density)            Ng^3 real
+     4      rhot (tracer density)        Ng^3 real
+     5      tracers per cell             Ng^3 int16
+     6      generated spectrum rhok      Ng*Ng*(Ng/2+1) complex (2 real each),
+            reference layout (kept only if option 1 is set)
+     7      log2 histogram               64 uint64                        */
+int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err);
+
+/* PDF of the last populate call: out[0]=nmax, [1]=ncell, [2]=ntot,
+   [3]=ncell before ties, [4]=ntot before ties, [5..5+nmax) cells per bin. */
+int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
+
+/* Options: 1 = keep a copy of the generated spectrum (tests);
+            2 = record per-kernel device times with HIP events. */
+int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
+
+/* Per-kernel device time totals (ms) and launch counts since the last reset
+   (option 2).  Returns the number of kernel classes; names are written as a
+   '\n'-separated list into `names` (capacity `cap`). */
+int EZmock_ext_kernel_times(EZMOCK *ez, double *ms, long *count, const int n,
+    char *names, const size_t cap);
+void EZmock_ext_reset_kernel_times(EZMOCK *ez);
+
+/* Wait for all device work of the instance. */
+int EZmock_ext_sync(EZMOCK *ez, int *err);
+
+/* ---- stand-alone device routines (parity tests) ---- */
+
+/* n uniform draws of stream `randgen` seeded with `seed`, starting at draw
+   index `start` (reset(seed, start) then get_double n times; `pos` selects
+   get_double_pos).  Computed on the GPU with per-thread jump-ahead. */
+int EZmock_ext_uniform(const int randgen, const uint64_t seed,
+    const uint64_t start, const size_t n, const bool pos, double *out,
+    int *err);
+
+/* 3-D complex-to-real FFT (unnormalised, e^{+ikx}) on the GPU.
+   in: Ng*Ng*(Ng/2+1) complex (interleaved), out: Ng^3 real. */
+int EZmock_ext_fft_c2r(const int Ng, const real *in, real *out, int *err);
+
+/* 3-D real-to-complex FFT (unnormalised, e^{-ikx}) on the GPU. */
+int EZmock_ext_fft_r2c(const int Ng, const real *in, real *out, int *err);
+
+/* PDF walk of pop_tracer.c:361-514 on the GPU for an arbitrary tracer-density
+   array: nt (int16, num) and the final pdf (nmax entries), ncell, ntot. */
+int EZmock_ext_pdf_walk(const real *rhot, const size_t num,
+    const size_t ncell, const size_t *pdf, const int nmax, int16_t *nt,
+    size_t *pdf_out, size_t *ncell_out, size_t *ntot_out, int *err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
