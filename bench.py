@@ -1,0 +1,211 @@
+"""Benchmark of the MI355X EZmock engine.
+
+One step = one complete mock of the single-mock drop-in workload
+(512^3, Lbox 2500 Mpc/h, z 0.5, MRG32k3a seed 42, bias (0.8, 8, 0.25, 200),
+2e-4 (h/Mpc)^3 -> 3,125,000 expected tracers, FP32): create_dens_field_from_ic
+followed by populate_tracer with the catalogue kept in device memory, with the
+linear P(k) table and cosmology already set (inputs resident on the device
+when the timed region starts).  Prints one JSON line on stdout.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+The roofline entry is for the dominant kernel of the timed region: its
+algorithmic bytes per launch (the per-unit model below x the units one launch
+processes) over its average launch time, measured with HIP events recorded on
+the engine's stream around every launch during the timed steps.
+The cpu_baseline entry times the CPU oracle (a C + OpenMP restatement of the
+reference path) on this host for one mock of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/mock at 512^3 & 1024^3 (FP32 vs FP64 GPU), batched mocks/s, p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, Ng, s, ntr):
+    """Algorithmic HBM bytes of one launch of a kernel class.  F = s N is
+    one real field; the half-complex spectrum is the same size."""
+    N = Ng ** 3
+    F = s * N
+    table = {
+        "gen_spectrum": F,                     # write the spectrum once
+        "fft_pass_i": 2 * F,                   # read + write the field
+        "fft_pass_j": 2 * F,
+        "fft_rows_c2r": 2 * F,
+        "zero_rho": F,
+        "cic_density": 4 * F,                  # read psi x3, write rho
+        "bias_count": F,
+        "bias": 2 * F,                         # read rho, write rhot
+        "compact": F,
+        "assign_nt": F + 2 * N,                # read rhot, write int16 counts
+        "positions": 2 * N + 3 * s * ntr,
+        "zero_mom": 3 * F,
+        "cic_momentum": 6 * F,                 # read psi x3, write 3 fields
+        "velocities": (32 + 6) * s * ntr,      # 8 corners x 4 fields + r/w
+    }
+    return table.get(kernel)
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=True):
+    from ezmock_amd import EZmock, inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = EZmock(Lbox=Lbox, Ngrid=Ng, rng=0, seed=seed, precision=precision)
+    ez.eval_growth_params(0.5, 0.31)
+    ez.setup_linear_pk(k, P)
+    nexp = int(round(density * Lbox ** 3))
+    ntr = 0
+    for _ in range(warmup):
+        ez.create_dens_field_from_ic()
+        ntr = ez.populate_tracer_device(*params, nexp)
+    ez.profile(profile)
+    ez.reset_kernel_times()
+    sync()
+    ez.sync()
+    t0 = time.perf_counter()
+    lat = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        ez.create_dens_field_from_ic()
+        ntr = ez.populate_tracer_device(*params, nexp)
+        lat.append(time.perf_counter() - t1)
+    ez.sync()
+    sync()
+    dt = time.perf_counter() - t0
+    kt = ez.kernel_times()
+    return dt, ntr, kt, lat
+
+
+def cpu_baseline(precision):
+    """The oracle on the host cores, one mock of the same workload."""
+    from oracle import pyoracle as po
+    from ezmock_amd import inputs
+    threads = min(os.cpu_count() or 1, 16)
+    k, P = inputs.eh_nowiggle_pk()
+    Ng, L = 512, 2500.
+    ez = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=42, nthread=threads, prec=precision)
+    ez.eval_growth_params(0.5, 0.31)
+    ez.setup_linear_pk(k, P)
+    t0 = time.perf_counter()
+    ez.create_dens_field_from_ic()
+    ez.populate_tracer(0.8, 8., 0.25, 200., int(round(2e-4 * L ** 3)))
+    dt = time.perf_counter() - t0
+    stages = {k_: round(v, 2) for k_, v in ez.stage_times().items()}
+    ez.close()
+    return {"value": round(dt * 1e3, 1), "unit": "ms/mock", "cores": threads,
+            "kind": "port",
+            "sample": f"one mock of the same workload (512^3, {precision}) by the oracle, a "
+                      f"C+OpenMP restatement of the reference CPU path with its own FFT, on "
+                      f"{threads} threads; stage seconds {stages}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary 1024^3 FP32 and 512^3 FP64 lines")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs one mock per GPU on a single GPU; runs on more "
+              "than one GPU are not part of this build (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    Ng, Lbox, seed = 512, 2500., 42
+    params = (0.8, 8., 0.25, 200.)
+    density = 2e-4
+    s = 4 if args.precision == "f32" else 8
+    log(f"bench: 512^3 {args.precision}, warmup {args.warmup}, steps {args.steps}")
+    dt, ntr, kt, lat = run_gpu(Ng, Lbox, seed, params, density, args.precision,
+                               args.steps, args.warmup)
+    ms = dt * 1e3 / args.steps
+    log(f"bench: {ms:.2f} ms/mock, {ntr} tracers")
+    for name, (t, c) in sorted(kt.items(), key=lambda x: -x[1][0]):
+        log(f"  {name:16s} {t / args.steps:9.3f} ms/mock  ({c} launches)")
+
+    # dominant kernel with a byte model
+    dom = max((x for x in kt.items() if algorithmic_bytes(x[0], Ng, s, ntr)),
+              key=lambda x: x[1][0])
+    name, (tot_ms, cnt) = dom
+    avg_s = tot_ms / cnt * 1e-3
+    bytes_ = algorithmic_bytes(name, Ng, s, ntr)
+    achieved = bytes_ / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(f"{name}_{Ng}_{args.precision}")
+        except Exception:
+            traffic = None
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "algorithmic_bytes": bytes_,
+            "avg_launch_ms": round(avg_s * 1e3, 4)}
+    total_alg = sum((algorithmic_bytes(n, Ng, s, ntr) or 0) * c for n, (t, c) in kt.items())
+    kernel_ms = sum(t for t, c in kt.values())
+    lat_ms = sorted(x * 1e3 for x in lat)
+    extra = {
+        "mock_latency_ms_p50": round(lat_ms[len(lat_ms) // 2], 2),
+        "mock_latency_ms_max": round(lat_ms[-1], 2),
+        "mocks_per_s": round(1e3 / ms, 3),
+        "tracers": ntr,
+        "all_kernels_alg_GBps": round(total_alg / (kernel_ms * 1e-3) / 1e9, 1),
+        "kernel_ms_per_mock": {n: round(t / args.steps, 3) for n, (t, c) in kt.items()},
+    }
+    if not args.no_extra:
+        try:
+            d2, n2, _, _ = run_gpu(1024, 2500., 42, params, density, "f32", 2, 1, profile=False)
+            extra["ms_per_mock_1024_f32"] = round(d2 * 1e3 / 2, 2)
+            log(f"bench: 1024^3 f32 {extra['ms_per_mock_1024_f32']} ms/mock")
+        except Exception as e:          # report, do not hide
+            extra["ms_per_mock_1024_f32"] = f"failed: {e}"
+        try:
+            d3, n3, _, _ = run_gpu(512, 2500., 42, params, density, "f64", 2, 1, profile=False)
+            extra["ms_per_mock_512_f64"] = round(d3 * 1e3 / 2, 2)
+            log(f"bench: 512^3 f64 {extra['ms_per_mock_512_f64']} ms/mock")
+        except Exception as e:
+            extra["ms_per_mock_512_f64"] = f"failed: {e}"
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the CPU oracle sample ...")
+        cpu = cpu_baseline(args.precision)
+        log(f"bench: cpu baseline {cpu['value']} ms/mock (scaled)")
+    line = {
+        "metric": METRIC, "value": round(ms, 3), "unit": "ms/mock", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic (EH no-wiggle linear P(k), seeded noise)",
+        "config": {"workload": "single mock 512^3, Lbox 2500 Mpc/h, z 0.5, MRG32k3a seed 42, "
+                               "bias (0.8, 8, 0.25, 200), 2e-4 (h/Mpc)^3",
+                   "Ng": Ng, "Lbox": Lbox, "nexp": int(round(density * Lbox ** 3)),
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

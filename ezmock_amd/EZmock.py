@@ -251,6 +251,18 @@ class EZmock:
         return dict(nmax=nmax, ncell=int(out[1]), ntot=int(out[2]), ncell0=int(out[3]),
                     ntot0=int(out[4]), pdf=out[5:5 + nmax].astype(np.int64))
 
+    def info(self):
+        o = np.zeros(8)
+        self._L.EZmock_ext_info(self._ez, o.ctypes.data)
+        return dict(Ng=int(o[0]), Lbox=o[1], growth2=o[2], vfac=o[3], pk_n=int(o[4]),
+                    kmin=o[5], kmax=o[6], pk_log=bool(o[7]))
+
+    def pk_table(self):
+        n = self.info()["pk_n"]
+        k, P, P2 = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._L.EZmock_ext_pk_table(self._ez, k.ctypes.data, P.ctypes.data, P2.ctypes.data, n)
+        return k, P, P2
+
     def set_option(self, option, value):
         return self._L.EZmock_ext_set_option(self._ez, int(option), int(value))
 

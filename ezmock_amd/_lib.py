@@ -25,7 +25,7 @@ EXT_SYMBOLS = [
     "EZmock_ext_copy_field", "EZmock_ext_pdf_info", "EZmock_ext_set_option",
     "EZmock_ext_kernel_times", "EZmock_ext_reset_kernel_times", "EZmock_ext_sync",
     "EZmock_ext_uniform", "EZmock_ext_fft_c2r", "EZmock_ext_fft_r2c",
-    "EZmock_ext_pdf_walk",
+    "EZmock_ext_pdf_walk", "EZmock_ext_info", "EZmock_ext_pk_table",
 ]
 
 
@@ -94,6 +94,10 @@ def load(precision="f32"):
     L.EZmock_ext_pdf_walk.restype = i32
     L.EZmock_ext_pdf_walk.argtypes = [vp, sz, sz, vp, i32, vp, vp, C.POINTER(sz),
                                       C.POINTER(sz), pint]
+    L.EZmock_ext_info.restype = i32
+    L.EZmock_ext_info.argtypes = [vp, vp]
+    L.EZmock_ext_pk_table.restype = i32
+    L.EZmock_ext_pk_table.argtypes = [vp, vp, vp, vp, i32]
     L._real = rt
     _CACHE[precision] = L
     return L

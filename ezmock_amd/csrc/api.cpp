@@ -1084,6 +1084,28 @@ void EZmock_ext_reset_kernel_times(EZMOCK *ez) {
   if (ez) MESH(ez)->times.clear();
 }
 
+int EZmock_ext_info(EZMOCK *ez, double *o) {
+  if (!ez || !o) return EZMOCK_ERR_ARG_NULL;
+  PkTab *pk = PK(ez);
+  o[0] = CONF(ez)->Ng; o[1] = CONF(ez)->Lbox;
+  o[2] = COSMO(ez)->growth2; o[3] = COSMO(ez)->vfac;
+  o[4] = pk ? pk->n : 0; o[5] = pk ? pk->kmin : 0; o[6] = pk ? pk->kmax : 0;
+  o[7] = pk ? pk->log : 0;
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_pk_table(EZMOCK *ez, double *k, double *P, double *P2,
+    const int n) {
+  if (!ez) return EZMOCK_ERR_ARG_EZ;
+  PkTab *pk = PK(ez);
+  if (!pk) return EZMOCK_ERR_ARG_PK;
+  if (n < pk->n || !k || !P || !P2) return EZMOCK_ERR_ARG_NBIN;
+  memcpy(k, pk->k.data(), sizeof(double) * pk->n);
+  memcpy(P, pk->P.data(), sizeof(double) * pk->n);
+  memcpy(P2, pk->P2.data(), sizeof(double) * pk->n);
+  return EZMOCK_SUCCESS;
+}
+
 int EZmock_ext_sync(EZMOCK *ez, int *err) {
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);

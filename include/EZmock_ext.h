@@ -64,6 +64,15 @@ int EZmock_ext_kernel_times(EZMOCK *ez, double *ms, long *count, const int n,
     char *names, const size_t cap);
 void EZmock_ext_reset_kernel_times(EZMOCK *ez);
 
+/* Host-side state: out[0] = Ng, [1] = Lbox, [2] = growth2, [3] = vfac,
+   [4] = number of P(k) table bins (0 if unset), [5] = kmin, [6] = kmax,
+   [7] = 1 if the spline is in log space. */
+int EZmock_ext_info(EZMOCK *ez, double *out8);
+
+/* The trimmed P(k) table and its spline second derivatives (n entries). */
+int EZmock_ext_pk_table(EZMOCK *ez, double *k, double *P, double *P2,
+    const int n);
+
 /* Wait for all device work of the instance. */
 int EZmock_ext_sync(EZMOCK *ez, int *err);
 

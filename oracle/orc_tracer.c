@@ -415,7 +415,10 @@ static int generate_tracers(orc_ez *ez, size_t ntot, bool att_part,
   if (bad) return ORC_ERR_UNKNOWN;
   /* pop_tracer.c:1012-1019 with disp_cic :593-683 */
   t0 = omp_get_wtime();
-  real *buf = ez->rhot;
+  /* The reference paints into the rhot buffer; a separate one keeps the
+     tracer density available to the tests. */
+  real *buf = malloc(num * sizeof(real));
+  if (!buf) return ORC_ERR_MEMORY;
   for (int d = 0; d < 3; d++) {
     memset(buf, 0, num * sizeof(real));
     orc_cic_paint(ez, d, buf);
@@ -425,6 +428,7 @@ static int generate_tracers(orc_ez *ez, size_t ntot, bool att_part,
     assign_vel(ez, buf, ntot - iv, pos[0] + iv, pos[1] + iv, pos[2] + iv,
         vel[d] + iv);
   }
+  free(buf);
   ez->tstage[ORC_ST_VEL] += omp_get_wtime() - t0;
   return 0;
 }

@@ -1,0 +1,73 @@
+"""Full-size GPU checks on the single-mock drop-in workload (512^3,
+Lbox 2500, seed 42, (0.8, 8, 0.25, 200), 2e-4 (h/Mpc)^3), through
+size-independent properties: mass conservation of the CIC field, PDF bin
+counts, rank order of the bins, tracers inside their host cells, the
+velocity scatter, and run-to-run determinism of the integer stages."""
+import numpy as np
+import pytest
+
+from ezmock_amd import EZmock
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mock512(pk_table):
+    k, P = pk_table
+    g = EZmock(Lbox=2500., Ngrid=512, rng=0, seed=42, precision="f32")
+    g.eval_growth_params(0.5, 0.31)
+    g.setup_linear_pk(k, P)
+    g.create_dens_field_from_ic()
+    n = g.populate_tracer_device(0.8, 8., 0.25, 200., 3125000)
+    return g, n
+
+
+def test_mass_conservation(gpu_available, mock512):
+    g, _ = mock512
+    rho = g.field("rho")
+    assert rho.sum(dtype=np.float64) == pytest.approx(512 ** 3, rel=1e-6)
+    assert rho.min() >= 0
+
+
+def test_pdf_counts_and_rank_order(gpu_available, mock512):
+    g, n = mock512
+    info = g.pdf_info()
+    assert info["nmax"] == 11
+    assert info["ncell0"] == 2343755 and info["ntot0"] == 3125006
+    assert info["ntot"] == n >= info["ntot0"]
+    nt = g.field("nt")
+    cnt = np.bincount(nt, minlength=info["nmax"] + 1)
+    assert list(cnt[1:][::-1]) == list(info["pdf"])
+    rhot = g.field("rhot")
+    prev_min = np.inf
+    for m in range(info["nmax"], 0, -1):
+        r = rhot[nt == m]
+        assert r.max() < prev_min or m == info["nmax"]
+        prev_min = r.min()
+
+
+def test_positions_in_cells_and_velocity_scatter(gpu_available, mock512):
+    g, n = mock512
+    x, y, z, vx, vy, vz = g.device_catalog()
+    assert len(x) == n
+    for a in (x, y, z):
+        assert a.min() >= 0 and a.max() <= 2500.
+    nt = g.field("nt").reshape(512, 512, 512)
+    gs = 2500. / 512
+    ci = np.floor(np.mod(np.stack([x, y, z], 1) / gs + 0.5, 512)).astype(np.int64)
+    flat = (ci[:, 0] * 512 + ci[:, 1]) * 512 + ci[:, 2]
+    cnt = np.bincount(flat, minlength=512 ** 3)
+    assert np.array_equal(cnt, nt.ravel())
+    v = np.concatenate([vx, vy, vz]).astype(np.float64)
+    assert np.isfinite(v).all()
+    assert 150 < v.std() < 2000
+
+
+def test_integer_stages_deterministic(gpu_available, mock512):
+    g, n = mock512
+    nt1 = g.field("nt").copy()
+    x1 = g.device_catalog()[0].copy()
+    n2 = g.populate_tracer_device(0.8, 8., 0.25, 200., 3125000)
+    assert n2 == n
+    assert np.array_equal(g.field("nt"), nt1)
+    assert np.array_equal(g.device_catalog()[0], x1)

@@ -1,0 +1,218 @@
+"""GPU parity tests: the HIP engine (through its C ABI) against the CPU
+oracle on the same seeded inputs, and against numpy / closed forms.
+
+Tolerances (stated per check): integer, index and stream work is bit-exact;
+spectra are bit-exact in FP32 and within a few ulp in FP64 (device vs glibc
+transcendentals); FFT outputs within FP rounding; the FP64 density field
+within 1e-10 relative and >= 99.99% identical tracer cells."""
+import os
+
+import numpy as np
+import pytest
+
+from ezmock_amd import EZmock, uniform, fft_c2r, fft_r2c, pdf_walk
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
+
+
+def test_uniform_draws_bit_exact(gpu_available):
+    for seed in (1, 42, 2 ** 40 + 3):
+        for start in (0, 1, 997, 5 * 512 ** 3 + 12345, 8 * 1536 ** 3 + 6):
+            g = uniform(seed, start, 3000)
+            o = po.Rng(0, seed, start, prec="f64").double(3000)
+            assert np.array_equal(g, o), (seed, start)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("Ng", [8, 12, 16, 24, 32, 48, 64, 96])
+def test_fft_against_numpy(gpu_available, Ng, prec):
+    rs = np.random.default_rng(Ng)
+    H = rs.standard_normal((Ng, Ng, Ng // 2 + 1)) + 1j * rs.standard_normal((Ng, Ng, Ng // 2 + 1))
+    ref = np.fft.irfftn(H, s=(Ng, Ng, Ng), axes=(0, 1, 2)) * Ng ** 3
+    tol = 1e-13 if prec == "f64" else 3e-6
+    assert np.abs(fft_c2r(H, prec) - ref).max() <= tol * np.abs(ref).max()
+    x = rs.standard_normal((Ng, Ng, Ng))
+    refk = np.fft.rfftn(x, axes=(0, 1, 2))
+    assert np.abs(fft_r2c(x, prec) - refk).max() <= tol * np.abs(refk).max()
+
+
+@pytest.mark.parametrize("Ng", [8, 12, 16])
+def test_fft_golden_vectors(gpu_available, Ng):
+    out = fft_c2r(GOLD[f"c2r_in_{Ng}"], "f64")
+    ref = GOLD[f"c2r_out_{Ng}"]
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+    out = fft_r2c(GOLD[f"r2c_in_{Ng}"], "f64")
+    ref = GOLD[f"r2c_out_{Ng}"]
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_pdf_walk_with_planted_ties(gpu_available, prec, seed):
+    rs = np.random.default_rng(seed)
+    n = 300000
+    rhot = rs.exponential(2.0, n)
+    rhot[rs.random(n) < 0.4] = 0
+    rhot = rhot.astype(po.rtype(prec))
+    e, info = po.eval_pdf(25000, 0.3 + 0.05 * seed, n, prec)
+    order = np.argsort(-rhot.astype(np.float64), kind="stable")
+    for c in np.cumsum(info["pdf"])[:-1]:
+        rhot[order[c:c + 2 + seed]] = rhot[order[c - 1]]
+    e1, nt1, p1, nc1, ntot1 = po.pdf_walk(rhot, info["ncell"], info["pdf"], prec)
+    e2, nt2, p2, nc2, ntot2 = pdf_walk(rhot, info["ncell"], info["pdf"], prec)
+    assert e1 == 0 and e2 == 0
+    assert np.array_equal(nt1, nt2)
+    assert np.array_equal(p1, p2) and (nc1, ntot1) == (nc2, ntot2)
+    assert nc1 > info["ncell"]
+
+
+def run_pair(Ng, L, prec, seed=1, fixamp=False, iphase=False, logpk=False, params=None,
+             nexp=None, pk=None, nthread=1):
+    k, P = pk
+    g = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=seed, precision=prec)
+    o = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=seed, nthread=nthread, prec=prec,
+                        keep_spec=True)
+    g.keep_spectrum(True)
+    for e in (g, o):
+        e.eval_growth_params(0.5, 0.31)
+        e.setup_linear_pk(k, P, interp_log=logpk)
+        e.create_dens_field_from_ic(fixamp=fixamp, iphase=iphase)
+    params = params or (0.5, 10., 0.3, 150.)
+    nexp = nexp or int(0.02 * Ng ** 3)
+    cg = g.populate_tracer(*params, nexp)
+    co = o.populate_tracer(*params, nexp)
+    return g, o, cg, co
+
+
+def rel_rms(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2))
+
+
+@pytest.mark.parametrize("Ng,prec,fixamp,iphase,logpk", [
+    (32, "f64", False, False, False), (32, "f32", False, False, False),
+    (48, "f64", True, False, False), (48, "f32", False, True, False),
+    (64, "f64", False, False, True), (64, "f32", True, True, True),
+])
+def test_pipeline_parity_small(gpu_available, pk_table, Ng, prec, fixamp, iphase, logpk):
+    g, o, cg, co = run_pair(Ng, 8.0 * Ng, prec, fixamp=fixamp, iphase=iphase, logpk=logpk,
+                            pk=pk_table)
+    sg, so = g.field("spec"), o.field("spec")
+    if prec == "f32":
+        # float spectra: the double->float rounding absorbs ulp-level device
+        # vs glibc differences except on rare rounding ties
+        assert np.mean(sg == so) > 0.9999
+        assert np.abs(sg.astype(float) - so).max() <= 2e-7 * np.abs(so).max()
+    else:
+        assert np.abs(sg - so).max() <= 4e-16 * np.abs(so).max()
+    for f in ("psi_x", "psi_y", "psi_z"):
+        assert rel_rms(g.field(f), o.field(f)) < (1e-13 if prec == "f64" else 1e-6)
+    rg, ro = g.field("rho"), o.field("rho")
+    if prec == "f64":
+        assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
+    else:
+        assert rel_rms(rg, ro) < 1e-5
+    pg, pO = g.pdf_info(), o.pdf_info()
+    assert pg["nmax"] == pO["nmax"]
+    ntg, nto = g.field("nt"), o.field("nt")
+    same = np.mean(ntg == nto)
+    assert same >= 0.9999
+    if same == 1.0:
+        assert (pg["ncell"], pg["ntot"]) == (pO["ncell"], pO["ntot"])
+        assert np.array_equal(pg["pdf"], pO["pdf"])
+        for a, b in zip(cg[:3], co[:3]):
+            assert np.array_equal(a, b)              # positions bit-exact
+        vt = 1e-9 if prec == "f64" else 2e-5
+        for a, b in zip(cg[3:], co[3:]):
+            assert np.abs(a.astype(float) - b).max() <= vt * max(1.0, np.abs(b).max())
+
+
+def test_white_noise_and_displacement_inputs(gpu_available, pk_table):
+    k, P = pk_table
+    Ng, L = 32, 256.
+    rs = np.random.default_rng(7)
+    delta = rs.standard_normal(Ng ** 3)
+    g = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=3, precision="f64")
+    o = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=3, nthread=1, prec="f64")
+    for e in (g, o):
+        e.eval_growth_params(0.5, 0.31)
+        e.setup_linear_pk(k, P)
+        e.create_dens_field_from_wn(delta)
+    for f in ("psi_x", "psi_y", "psi_z", "rho"):
+        assert rel_rms(g.field(f), o.field(f)) < 1e-12, f
+    # displacement input, deep copy and by reference
+    psi = [o.field(f) for f in ("psi_x", "psi_y", "psi_z")]
+    for deep in (True, False):
+        g2 = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=3, precision="f64")
+        g2.eval_growth_params(0.5, 0.31)
+        g2.create_dens_field_from_disp(*psi, deepcopy=deep)
+        assert np.abs(g2.field("rho") - o.field("rho")).max() < 1e-10
+        cg = g2.populate_tracer(0.4, 8., 0.3, 100., 600)
+        co = o.populate_tracer(0.4, 8., 0.3, 100., 600)
+        assert np.array_equal(cg[0], co[0])
+
+
+def test_repeated_populate_and_determinism(gpu_available, pk_table):
+    k, P = pk_table
+    g = EZmock(Lbox=320., Ngrid=32, rng=0, seed=9, precision="f64")
+    g.eval_growth_params(0.5, 0.31)
+    g.setup_linear_pk(k, P)
+    g.create_dens_field_from_ic()
+    a = g.populate_tracer(0.5, 10., 0.3, 150., 700)
+    b = g.populate_tracer(0.7, 6., 0.25, 50., 900)
+    c = g.populate_tracer(0.5, 10., 0.3, 150., 700)
+    for x, y in zip(a, c):
+        assert np.array_equal(x, y)
+    assert len(b[0]) != len(a[0]) or not np.array_equal(a[0], b[0])
+
+
+def test_device_rsd_matches_reference_formula(gpu_available, pk_table):
+    k, P = pk_table
+    g = EZmock(Lbox=320., Ngrid=32, rng=0, seed=9, precision="f32")
+    g.eval_growth_params(0.5, 0.31)
+    g.setup_linear_pk(k, P)
+    g.create_dens_field_from_ic()
+    g.populate_tracer_device(0.5, 10., 0.3, 150., 700)
+    x, y, z, vx, vy, vz = g.device_catalog()
+    fac = 0.011384
+    zr = g.rsd_z(fac)
+    ref = z.astype(np.float64) + fac * vz.astype(np.float64)
+    ref = np.where(ref >= 320., ref - 320., ref)
+    ref = np.where(ref < 0, ref + 320., ref)
+    assert np.array_equal(zr, ref)
+
+
+def test_unsupported_configurations_fail_loudly(gpu_available, pk_table):
+    k, P = pk_table
+    g = EZmock(Lbox=320., Ngrid=40, rng=0)
+    g.eval_growth_params(0.5, 0.31)
+    g.setup_linear_pk(k, P)
+    with pytest.raises(RuntimeError, match="invalid grid size"):
+        g.create_dens_field_from_ic()
+    g = EZmock(Lbox=320., Ngrid=32, rng=1)
+    g.eval_growth_params(0.5, 0.31)
+    g.setup_linear_pk(k, P)
+    with pytest.raises(RuntimeError, match="failed to setup the random generator"):
+        g.create_dens_field_from_ic()
+
+
+def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
+    """Plain EZmock single mock, FP64, as the reference CPU path runs it:
+    256^3, Lbox 1000, z 0.5, MRG32k3a seed 1, (0.5, 10, 0.3, 150),
+    3e-4 (h/Mpc)^3 -> 300000 expected tracers."""
+    g, o, cg, co = run_pair(256, 1000., "f64", seed=1, params=(0.5, 10., 0.3, 150.),
+                            nexp=300000, pk=pk_table, nthread=os.cpu_count() or 1)
+    rg, ro = g.field("rho"), o.field("rho")
+    assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
+    ntg, nto = g.field("nt"), o.field("nt")
+    same = np.mean(ntg == nto)
+    assert same >= 0.9999
+    pg, pO = g.pdf_info(), o.pdf_info()
+    assert abs(pg["ntot"] - pO["ntot"]) <= max(3, 1e-4 * pO["ntot"])
+    if same == 1.0:
+        for a, b in zip(cg[:3], co[:3]):
+            assert np.array_equal(a, b)
+        for a, b in zip(cg[3:], co[3:]):
+            assert np.abs(a - b).max() < 1e-6
