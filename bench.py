@@ -44,15 +44,16 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
         "fft_pass_i": 2 * F,                   # read + write the field
         "fft_pass_j": 2 * F,
         "fft_rows_c2r": 2 * F,
-        "zero_rho": F,
-        "cic_density": 4 * F,                  # read psi x3, write rho
         "bias_count": F,
         "bias": 2 * F,                         # read rho, write rhot
         "compact": F,
         "assign_nt": F + 2 * N,                # read rhot, write int16 counts
         "positions": 2 * N + 3 * s * ntr,
-        "zero_mom": 3 * F,
-        "cic_momentum": 6 * F,                 # read psi x3, write 3 fields
+        "cic_bin_count": 3 * F,                # read psi x3
+        "cic_bin_scatter": 3 * F + 4 * N,      # read psi x3, write indices
+        "cic_tiles_rho_mom": 7 * F,            # read psi x3, write 4 fields
+        "cic_tiles_rho": 4 * F,
+        "cic_tiles_mom": 6 * F,
         "velocities": (32 + 6) * s * ntr,      # 8 corners x 4 fields + r/w
     }
     return table.get(kernel)

@@ -52,6 +52,10 @@ struct Mesh {
   int16_t *nt = nullptr;
   real *mom[3] = {nullptr, nullptr, nullptr};
   bool mom_valid = false;
+  // CIC binning scratch
+  uint32_t *cic_counts = nullptr, *cic_list = nullptr;
+  uint64_t *cic_off = nullptr, *cic_cur = nullptr, *cic_scan = nullptr;
+  size_t cic_list_cap = 0;
   double *ptab = nullptr;
   long ptab_version = -1;
   cplx *twN = nullptr, *twh = nullptr;
@@ -149,6 +153,8 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
   dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
   dfree(&m->hist); dfree(&m->counter);
+  dfree(&m->cic_counts); dfree(&m->cic_list); m->cic_list_cap = 0; dfree(&m->cic_off);
+  dfree(&m->cic_cur); dfree(&m->cic_scan);
   free_tracer_scratch(m);
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
@@ -218,6 +224,52 @@ int ensure_twiddles(Mesh *m, int Ng) {
   }
   HIP_OK(hipMemcpy(m->twN, a.data(), sizeof(cplx) * Ng, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(m->twh, b.data(), sizeof(cplx) * Ngh, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// CIC painting of the displaced particles (cic.hip): density into rho
+// (unless null) and, when with_mom, the three momentum fields.
+int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
+  const CicGeom g = cic_geometry(Ng, igs);
+  if (g.N >= 0xffffffffull) {
+    fprintf(stderr, "EZmock: grids of 2^32 cells or more are not supported\n");
+    return EZMOCK_ERR_ARG_NGRID;
+  }
+  const size_t nk = cic_nkeys(g);
+  int e;
+  if ((e = dalloc(&m->cic_counts, nk)) || (e = dalloc(&m->cic_off, nk + 1)) ||
+      (e = dalloc(&m->cic_cur, nk)) || (e = dalloc(&m->cic_scan, scan_tmp_words(nk) + 8)))
+    return e;
+  if (with_mom)
+    for (int i = 0; i < 3; i++)
+      if ((e = dalloc(&m->mom[i], g.N))) return e;
+  {
+    Scope s(m, "cic_bin_count");
+    KCHK(launch_cic_count(m->psi[0], m->psi[1], m->psi[2], g, m->cic_counts,
+        m->cic_off, m->cic_cur, m->cic_scan, m->st));
+  }
+  uint64_t total = 0;
+  HIP_OK(hipMemcpyAsync(&total, m->cic_off + nk, sizeof(uint64_t),
+      hipMemcpyDeviceToHost, m->st));
+  if ((e = sync_and_collect(m))) return e;
+  if (total < g.N || total > 8 * g.N) return EZMOCK_ERR_UNKNOWN;
+  if (m->cic_list_cap < total) {
+    dfree(&m->cic_list);
+    const size_t cap = total + total / 8;
+    if ((e = dalloc(&m->cic_list, cap))) { m->cic_list_cap = 0; return e; }
+    m->cic_list_cap = cap;
+  }
+  {
+    Scope s(m, "cic_bin_scatter");
+    KCHK(launch_cic_scatter(m->psi[0], m->psi[1], m->psi[2], g, m->cic_cur,
+        m->cic_list, m->st));
+  }
+  {
+    Scope s(m, with_mom ? (rho ? "cic_tiles_rho_mom" : "cic_tiles_mom") : "cic_tiles_rho");
+    KCHK(launch_cic_tiles(m->psi[0], m->psi[1], m->psi[2], g, m->cic_off,
+        m->cic_list, rho, with_mom ? m->mom[0] : nullptr,
+        with_mom ? m->mom[1] : nullptr, with_mom ? m->mom[2] : nullptr, m->st));
+  }
   return 0;
 }
 
@@ -454,17 +506,9 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   }
   // ---- velocities, pop_tracer.c:593-753 and :1043-1113
   if (!m->mom_valid) {
-    for (int i = 0; i < 3; i++)
-      if ((e = dalloc(&m->mom[i], N))) return e;
-    {
-      Scope s(m, "zero_mom");
-      for (int i = 0; i < 3; i++) KCHK(launch_zero(m->mom[i], N * sizeof(real), m->st));
-    }
-    {
-      Scope s(m, "cic_momentum");
-      KCHK(launch_cic(m->psi[0], m->psi[1], m->psi[2], m->mom[0], m->mom[1],
-          m->mom[2], Ng, igs, 1, m->st));
-    }
+    // fields given by reference: the momentum fields follow the caller's
+    // current arrays, the density stays the one of create_dens_field
+    if ((e = paint_cic(m, Ng, igs, nullptr, true))) return e;
     m->mom_valid = !m->psi_ref;
   }
   {
@@ -811,15 +855,11 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
       for (int c = 0; c < 3; c++) KCHK(launch_scale(m->psi[c], norm, N, m->st));
     }
   }
-  {
-    Scope s(m, "zero_rho");
-    KCHK(launch_zero(m->rho, N * sizeof(real), m->st));
-  }
-  {
-    Scope s(m, "cic_density");
-    KCHK(launch_cic(m->psi[0], m->psi[1], m->psi[2], m->rho, nullptr, nullptr,
-        Ng, (real) (conf->Ng / conf->Lbox), 0, m->st));
-  }
+  // density and (for fields owned by the engine) the momentum fields used
+  // by every later populate call
+  if ((e = paint_cic(m, Ng, (real) (conf->Ng / conf->Lbox), m->rho, !m->psi_ref)))
+    return (*err = e);
+  m->mom_valid = !m->psi_ref;
   if ((e = sync_and_collect(m))) return (*err = e);
   m->fixamp = fixamp;
   m->iphase = iphase;

@@ -3,11 +3,7 @@
 //   k_pk_table    P(k) at every integer |n|^2 of the grid (pk_interp of
 //                 linear_pk.c:194-202 evaluated once per distinct |k|).
 //   k_gen_rhok    noise + Zel'dovich spectrum, perturb.c:78-359.
-//   k_cic         CIC mass assignment, dens_field.c:233-310, and the
-//                 momentum painting of pop_tracer.c:593-675 (3 components
-//                 in one pass over the particles).
-// This file is compiled with -ffp-contract=off: the particle coordinates and
-// CIC weights are formed with the same IEEE operations as the reference.
+// (CIC painting is in cic.hip.)  Compiled with -ffp-contract=off.
 #include "common.h"
 #include "rng.h"
 #include "kernels.h"
@@ -148,89 +144,6 @@ hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
     double logkfac, hipStream_t st) {
   hipLaunchKernelGGL(k_pk_table, dim3(ez_blocks(nsq, 256)), dim3(256), 0, st,
       tab, nsq, kt, Pt, P2t, n, logpk, kfac, logkfac);
-  return hipGetLastError();
-}
-
-// ------------------------------------------------------------------ CIC
-__device__ __forceinline__ void atomic_add_real(real *p, real v) {
-  unsafeAtomicAdd(p, v);
-}
-
-// One thread per particle (Lagrangian cell), k fastest.  MOM = 0: density
-// (dens_field.c:260-306); MOM = 1: the three momentum fields psi_a * w
-// (pop_tracer.c:626-671, weights scaled through wx0/wx1 as there).
-template <int MOM>
-__global__ void __launch_bounds__(256)
-k_cic(const real *__restrict__ p0, const real *__restrict__ p1,
-    const real *__restrict__ p2, real *__restrict__ o0, real *__restrict__ o1,
-    real *__restrict__ o2, int Ng, real igs, size_t N) {
-  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N) return;
-  const int k = (int) (idx % Ng);
-  const size_t ij = idx / Ng;
-  const int j = (int) (ij % Ng);
-  const int i = (int) (ij / Ng);
-  const real q0 = p0[idx], q1 = p1[idx], q2 = p2[idx];
-  const real x = q0 * igs + (real) i;
-  const real y = q1 * igs + (real) j;
-  const real z = q2 * igs + (real) k;
-  int x0 = (int) floor((double) x), y0 = (int) floor((double) y),
-      z0 = (int) floor((double) z);
-  const real wx1 = x - (real) x0, wx0 = 1 - wx1;
-  const real wy1 = y - (real) y0, wy0 = 1 - wy1;
-  const real wz1 = z - (real) z0, wz0 = 1 - wz1;
-  if (x0 < 0) x0 += Ng; else if (x0 >= Ng) x0 -= Ng;
-  if (y0 < 0) y0 += Ng; else if (y0 >= Ng) y0 -= Ng;
-  if (z0 < 0) z0 += Ng; else if (z0 >= Ng) z0 -= Ng;
-  // Displacements beyond one box length (or non-finite ones) are undefined
-  // behaviour in the reference; keep every index inside the grid.
-  if ((unsigned) x0 >= (unsigned) Ng) x0 = ((x0 % Ng) + Ng) % Ng;
-  if ((unsigned) y0 >= (unsigned) Ng) y0 = ((y0 % Ng) + Ng) % Ng;
-  if ((unsigned) z0 >= (unsigned) Ng) z0 = ((z0 % Ng) + Ng) % Ng;
-  const int x1 = (x0 == Ng - 1) ? 0 : x0 + 1;
-  const int y1 = (y0 == Ng - 1) ? 0 : y0 + 1;
-  const int z1 = (z0 == Ng - 1) ? 0 : z0 + 1;
-  const size_t a00 = ((size_t) x0 * Ng + y0) * Ng, a01 = ((size_t) x0 * Ng + y1) * Ng;
-  const size_t a10 = ((size_t) x1 * Ng + y0) * Ng, a11 = ((size_t) x1 * Ng + y1) * Ng;
-  if (MOM == 0) {
-    atomic_add_real(o0 + a00 + z0, wx0 * wy0 * wz0);
-    atomic_add_real(o0 + a00 + z1, wx0 * wy0 * wz1);
-    atomic_add_real(o0 + a01 + z0, wx0 * wy1 * wz0);
-    atomic_add_real(o0 + a01 + z1, wx0 * wy1 * wz1);
-    atomic_add_real(o0 + a10 + z0, wx1 * wy0 * wz0);
-    atomic_add_real(o0 + a10 + z1, wx1 * wy0 * wz1);
-    atomic_add_real(o0 + a11 + z0, wx1 * wy1 * wz0);
-    atomic_add_real(o0 + a11 + z1, wx1 * wy1 * wz1);
-  }
-  else {
-    const real qs[3] = {q0, q1, q2};
-    real *os[3] = {o0, o1, o2};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const real u0 = wx0 * qs[d], u1 = wx1 * qs[d];
-      real *o = os[d];
-      atomic_add_real(o + a00 + z0, u0 * wy0 * wz0);
-      atomic_add_real(o + a00 + z1, u0 * wy0 * wz1);
-      atomic_add_real(o + a01 + z0, u0 * wy1 * wz0);
-      atomic_add_real(o + a01 + z1, u0 * wy1 * wz1);
-      atomic_add_real(o + a10 + z0, u1 * wy0 * wz0);
-      atomic_add_real(o + a10 + z1, u1 * wy0 * wz1);
-      atomic_add_real(o + a11 + z0, u1 * wy1 * wz0);
-      atomic_add_real(o + a11 + z1, u1 * wy1 * wz1);
-    }
-  }
-}
-
-hipError_t launch_cic(const real *p0, const real *p1, const real *p2,
-    real *o0, real *o1, real *o2, int Ng, real igs, int momentum,
-    hipStream_t st) {
-  const size_t N = (size_t) Ng * Ng * Ng;
-  if (momentum)
-    hipLaunchKernelGGL(k_cic<1>, dim3(ez_blocks(N, 256)), dim3(256), 0, st,
-        p0, p1, p2, o0, o1, o2, Ng, igs, N);
-  else
-    hipLaunchKernelGGL(k_cic<0>, dim3(ez_blocks(N, 256)), dim3(256), 0, st,
-        p0, p1, p2, o0, o1, o2, Ng, igs, N);
   return hipGetLastError();
 }
 

@@ -22,11 +22,27 @@ hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     const cplx *dk_ny, const GenParams &gp, const double *ptab,
     real *ref_copy, hipStream_t st);
 hipError_t launch_unpack_r2c(cplx *buf, cplx *ny2, int Ng, hipStream_t st);
-hipError_t launch_cic(const real *p0, const real *p1, const real *p2,
-    real *o0, real *o1, real *o2, int Ng, real igs, int momentum,
-    hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st);
+
+// cic.hip
+struct CicGeom {
+  int Ng;
+  real igs;
+  int TX, TY, TZ;               // tile shape in cells
+  int ntx, nty, ntz;            // tiles per dimension
+  size_t N;
+};
+CicGeom cic_geometry(int Ng, real igs);
+size_t cic_nkeys(const CicGeom &g);
+hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
+    const CicGeom &g, uint32_t *counts, uint64_t *off, uint64_t *cursor,
+    uint64_t *scan_tmp, hipStream_t st);
+hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
+    const CicGeom &g, uint64_t *cursor, uint32_t *list, hipStream_t st);
+hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,
+    const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
+    real *m0, real *m1, real *m2, hipStream_t st);
 
 // tracer.hip
 struct BiasParams {

@@ -247,28 +247,40 @@ __device__ __forceinline__ rkey real_key(real v) {
 #endif
 }
 
+// One device atomic per block of 4096 cells reserves the block's range.
 __global__ void __launch_bounds__(TPB)
 k_compact(const real *__restrict__ rhot, real thres, size_t N,
     rkey *__restrict__ keys, size_t cap, unsigned long long *__restrict__ counter) {
-  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  const bool take = (i < N) && rhot[i] >= thres;
-  const uint64_t m = __ballot(take);
-  if (!m) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long) m) - 1;
-  unsigned long long b = 0;
-  if (lane == leader) b = atomicAdd(counter, (unsigned long long) __popcll(m));
-  b = __shfl(b, leader, 64);
-  if (take) {
-    const uint64_t below = m & ((1ull << lane) - 1);
-    const size_t pos = b + __popcll(below);
-    if (pos < cap) keys[pos] = real_key(rhot[i]);
+  __shared__ uint32_t ws[TPB / 64];
+  __shared__ unsigned long long base;
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  real v[CPT];
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    v[q] = (i < N) ? rhot[i] : (real) 0;
+    if (i < N && v[q] >= thres) c++;
+  }
+  uint32_t tot;
+  const uint32_t excl = block_excl_scan<uint32_t>(c, tot, ws);
+  if (threadIdx.x == 0 && tot) base = atomicAdd(counter, (unsigned long long) tot);
+  __syncthreads();
+  if (!c) return;
+  size_t pos = base + excl;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    if (i < N && v[q] >= thres) {
+      if (pos < cap) keys[pos] = real_key(v[q]);
+      pos++;
+    }
   }
 }
 
 hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
     size_t cap, unsigned long long *counter, hipStream_t st) {
-  hipLaunchKernelGGL(k_compact, dim3(ez_blocks(N, TPB)), dim3(TPB), 0, st,
+  hipLaunchKernelGGL(k_compact, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
       rhot, thres, N, keys, cap, counter);
   return hipGetLastError();
 }
