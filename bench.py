@@ -51,9 +51,9 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
         "positions": 2 * N + 3 * s * ntr,
         "cic_bin_count": 3 * F,                # read psi x3
         "cic_bin_scatter": 3 * F + 4 * N,      # read psi x3, write indices
-        "cic_tiles_rho_mom": 7 * F,            # read psi x3, write 4 fields
+        "cic_tiles_rho_mom": 8 * F,            # read psi x3, write rho + {rho, u x3}
         "cic_tiles_rho": 4 * F,
-        "cic_tiles_mom": 6 * F,
+        "cic_tiles_mom": 8 * F,                # read psi x3 + rho, write {rho, u x3}
         "velocities": (32 + 6) * s * ntr,      # 8 corners x 4 fields + r/w
     }
     return table.get(kernel)

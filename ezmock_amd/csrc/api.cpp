@@ -50,7 +50,7 @@ struct Mesh {
   cplx *ny = nullptr;           // kz = Nyquist plane (+ r2c scratch), 2 Ng^2
   real *rhot = nullptr;
   int16_t *nt = nullptr;
-  real *mom[3] = {nullptr, nullptr, nullptr};
+  real4 *vel4 = nullptr;        // {rho, u0, u1, u2} per cell, for the velocities
   bool mom_valid = false;
   // CIC binning scratch
   uint32_t *cic_counts = nullptr, *cic_list = nullptr;
@@ -148,7 +148,8 @@ void free_tracer_scratch(Mesh *m) {
 }
 
 void free_mesh_device(Mesh *m) {
-  for (int i = 0; i < 3; i++) { dfree(&m->psi[i]); dfree(&m->mom[i]); }
+  for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
+  dfree(&m->vel4);
   dfree(&m->rho); dfree(&m->ny); dfree(&m->rhot); dfree(&m->nt);
   dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
   dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
@@ -240,9 +241,7 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   if ((e = dalloc(&m->cic_counts, nk)) || (e = dalloc(&m->cic_off, nk + 1)) ||
       (e = dalloc(&m->cic_cur, nk)) || (e = dalloc(&m->cic_scan, scan_tmp_words(nk) + 8)))
     return e;
-  if (with_mom)
-    for (int i = 0; i < 3; i++)
-      if ((e = dalloc(&m->mom[i], g.N))) return e;
+  if (with_mom && (e = dalloc(&m->vel4, g.N))) return e;
   {
     Scope s(m, "cic_bin_count");
     KCHK(launch_cic_count(m->psi[0], m->psi[1], m->psi[2], g, m->cic_counts,
@@ -267,8 +266,7 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   {
     Scope s(m, with_mom ? (rho ? "cic_tiles_rho_mom" : "cic_tiles_mom") : "cic_tiles_rho");
     KCHK(launch_cic_tiles(m->psi[0], m->psi[1], m->psi[2], g, m->cic_off,
-        m->cic_list, rho, with_mom ? m->mom[0] : nullptr,
-        with_mom ? m->mom[1] : nullptr, with_mom ? m->mom[2] : nullptr, m->st));
+        m->cic_list, rho, m->rho, with_mom ? m->vel4 : nullptr, m->st));
   }
   return 0;
 }
@@ -513,8 +511,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   }
   {
     Scope s(m, "velocities");
-    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->rho, m->mom[0],
-        m->mom[1], m->mom[2], ntot, Ng, igs, cosmo->vfac, (double) sigma_v,
+    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, Ng, igs, cosmo->vfac, (double) sigma_v,
         bp.seed_state, m->jtab, m->cat[3], m->cat[4], m->cat[5], m->st));
   }
   if ((e = sync_and_collect(m))) return e;

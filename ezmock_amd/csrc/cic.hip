@@ -33,34 +33,17 @@ struct ParticleCell {
   int x0, y0, z0;               // base cell (floor), before wrapping
 };
 
-// dens_field.c:263-289
-__device__ __forceinline__ ParticleCell particle_cell(const real *__restrict__ p0,
-    const real *__restrict__ p1, const real *__restrict__ p2, size_t idx,
-    int Ng, real igs) {
-  ParticleCell c;
-  const int k = (int) (idx % Ng);
-  const size_t ij = idx / Ng;
-  const int j = (int) (ij % Ng);
-  const int i = (int) (ij / Ng);
-  c.x = p0[idx] * igs + (real) i;
-  c.y = p1[idx] * igs + (real) j;
-  c.z = p2[idx] * igs + (real) k;
-  int x0 = (int) floor((double) c.x), y0 = (int) floor((double) c.y),
-      z0 = (int) floor((double) c.z);
-  c.x0 = x0; c.y0 = y0; c.z0 = z0;      // unwrapped: weights use these
-  return c;
-}
-
+// dens_field.c:263-289: grid coordinate of particle idx (Lagrangian cell
+// (i, j, k), idx = (i Ng + j) Ng + k) displaced by q / cell size.
 __device__ __forceinline__ ParticleCell cell_from(real q0, real q1, real q2,
-    size_t idx, int Ng, real igs) {
+    uint32_t idx, const CicGeom &g) {
   ParticleCell c;
-  const int k = (int) (idx % Ng);
-  const size_t ij = idx / Ng;
-  const int j = (int) (ij % Ng);
-  const int i = (int) (ij / Ng);
-  c.x = q0 * igs + (real) i;
-  c.y = q1 * igs + (real) j;
-  c.z = q2 * igs + (real) k;
+  uint32_t k, j;
+  const uint32_t ij = fastdiv_qr(idx, g.fNg, k);
+  const uint32_t i = fastdiv_qr(ij, g.fNg, j);
+  c.x = q0 * g.igs + (real) (int) i;
+  c.y = q1 * g.igs + (real) (int) j;
+  c.z = q2 * g.igs + (real) (int) k;
   c.x0 = (int) floor((double) c.x);
   c.y0 = (int) floor((double) c.y);
   c.z0 = (int) floor((double) c.z);
@@ -76,29 +59,40 @@ __device__ __forceinline__ int wrap_once(int a, int Ng) {
   return a;
 }
 
-// Tiles touched by the stencil: tx0 and (if the base cell is the last of
-// its tile) tx0 + 1, and likewise in y and z.  Returns the count; keys are
-// tile ids (tx * nty + ty) * ntz + tz.
-__device__ __forceinline__ int tiles_of(const ParticleCell &c, const CicGeom &g,
-    uint32_t *keys) {
+// Tiles touched by the stencil of a particle: base tile (tx, ty, tz) of the
+// wrapped base cell, and the next tile along each axis where the base cell
+// is the last of its tile (e = 2).  The n = ex ey ez tiles are enumerated by
+// v = (a ey + b) ez + d.
+struct TileSpan {
+  int tx, ty, tz;
+  int ex, ey, ez;
+};
+
+__device__ __forceinline__ TileSpan tile_span(const ParticleCell &c, const CicGeom &g) {
+  TileSpan s;
   const int x0 = wrap_once(c.x0, g.Ng), y0 = wrap_once(c.y0, g.Ng),
       z0 = wrap_once(c.z0, g.Ng);
-  const int tx = x0 / g.TX, ty = y0 / g.TY, tz = z0 / g.TZ;
-  const int ex = (x0 - tx * g.TX == g.TX - 1 && g.ntx > 1) ? 2 : 1;
-  const int ey = (y0 - ty * g.TY == g.TY - 1 && g.nty > 1) ? 2 : 1;
-  const int ez = (z0 - tz * g.TZ == g.TZ - 1 && g.ntz > 1) ? 2 : 1;
-  int n = 0;
-  for (int a = 0; a < ex; a++) {
-    const int ux = (tx + a == g.ntx) ? 0 : tx + a;
-    for (int b = 0; b < ey; b++) {
-      const int uy = (ty + b == g.nty) ? 0 : ty + b;
-      for (int d = 0; d < ez; d++) {
-        const int uz = (tz + d == g.ntz) ? 0 : tz + d;
-        keys[n++] = (uint32_t) ((ux * g.nty + uy) * g.ntz + uz);
-      }
-    }
-  }
-  return n;
+  s.tx = x0 >> g.lx; s.ty = y0 >> g.ly; s.tz = z0 >> g.lz;
+  s.ex = ((x0 & (g.TX - 1)) == g.TX - 1 && g.ntx > 1) ? 2 : 1;
+  s.ey = ((y0 & (g.TY - 1)) == g.TY - 1 && g.nty > 1) ? 2 : 1;
+  s.ez = ((z0 & (g.TZ - 1)) == g.TZ - 1 && g.ntz > 1) ? 2 : 1;
+  return s;
+}
+
+// Tile coordinates of the v-th tile of a span (v < ex ey ez).
+__device__ __forceinline__ void span_tile(const TileSpan &s, const CicGeom &g,
+    int v, int &ux, int &uy, int &uz) {
+  int d = 0, b = 0;
+  if (s.ez == 2) { d = v & 1; v >>= 1; }
+  if (s.ey == 2) { b = v & 1; v >>= 1; }
+  ux = s.tx + v; uy = s.ty + b; uz = s.tz + d;
+  if (ux == g.ntx) ux = 0;
+  if (uy == g.nty) uy = 0;
+  if (uz == g.ntz) uz = 0;
+}
+
+__device__ __forceinline__ uint32_t tile_key(const CicGeom &g, int ux, int uy, int uz) {
+  return (uint32_t) ((ux * g.nty + uy) * g.ntz + uz);
 }
 
 // Local tile window of a workgroup of CIC_PPB consecutive particles: the
@@ -112,35 +106,37 @@ struct Window {
   int txlo, tylo, wx, wy;       // window origin and extent in tiles
 };
 
-__device__ __forceinline__ Window make_window(const CicGeom &g, size_t b0) {
+__device__ __forceinline__ Window make_window(const CicGeom &g, uint32_t b0) {
   Window w;
   const size_t ntile = (size_t) g.ntx * g.nty * g.ntz;
   w.direct = ntile <= WMAX;
-  const size_t plane = (size_t) g.Ng * g.Ng;
-  const size_t pl = ((b0 + CIC_PPB < g.N) ? b0 + CIC_PPB : g.N) - 1;
-  const int i0 = (int) (b0 / plane), i1 = (int) (pl / plane);
+  const uint32_t pl = (uint32_t) (((size_t) b0 + CIC_PPB < g.N) ? (size_t) b0 + CIC_PPB
+                                                                 : g.N) - 1;
+  uint32_t r;
+  const uint32_t r0 = fastdiv_qr(b0, g.fNg, r), r1 = fastdiv_qr(pl, g.fNg, r);
+  uint32_t j0u, j1u;
+  const int i0 = (int) fastdiv_qr(r0, g.fNg, j0u), i1 = (int) fastdiv_qr(r1, g.fNg, j1u);
   int j0 = 0, j1 = g.Ng - 1;
-  if (i0 == i1) { j0 = (int) ((b0 / g.Ng) % g.Ng); j1 = (int) ((pl / g.Ng) % g.Ng); }
-  w.wx = i1 / g.TX - i0 / g.TX + 3;
-  w.wy = j1 / g.TY - j0 / g.TY + 3;
+  if (i0 == i1) { j0 = (int) j0u; j1 = (int) j1u; }
+  w.wx = (i1 >> g.lx) - (i0 >> g.lx) + 3;
+  w.wy = (j1 >> g.ly) - (j0 >> g.ly) + 3;
   if (w.wx > g.ntx) w.wx = g.ntx;
   if (w.wy > g.nty) w.wy = g.nty;
-  w.txlo = (i0 / g.TX - 1 + g.ntx) % g.ntx;
-  w.tylo = (j0 / g.TY - 1 + g.nty) % g.nty;
+  w.txlo = (i0 >> g.lx) - 1; if (w.txlo < 0) w.txlo += g.ntx;
+  w.tylo = (j0 >> g.ly) - 1; if (w.tylo < 0) w.tylo += g.nty;
   if (!w.direct && (size_t) w.wx * w.wy * g.ntz > WMAX) {
     w.wx = 1; w.wy = 1;         // degenerate shapes: mostly global path
   }
   return w;
 }
 
-// Window slot of a tile key, or -1 outside the window.
+// Window slot of tile (ux, uy, uz), or -1 outside the window.
 __device__ __forceinline__ int window_slot(const Window &w, const CicGeom &g,
-    uint32_t key) {
-  if (w.direct) return (int) key;
-  const int uz = (int) (key % g.ntz);
-  const int uy = (int) ((key / g.ntz) % g.nty);
-  const int ux = (int) (key / ((uint32_t) g.ntz * g.nty));
-  const int dx = (ux - w.txlo + g.ntx) % g.ntx, dy = (uy - w.tylo + g.nty) % g.nty;
+    int ux, int uy, int uz) {
+  if (w.direct) return (int) tile_key(g, ux, uy, uz);
+  int dx = ux - w.txlo, dy = uy - w.tylo;
+  if (dx < 0) dx += g.ntx;
+  if (dy < 0) dy += g.nty;
   if (dx >= w.wx || dy >= w.wy) return -1;
   return (dx * w.wy + dy) * g.ntz + uz;
 }
@@ -148,10 +144,51 @@ __device__ __forceinline__ int window_slot(const Window &w, const CicGeom &g,
 __device__ __forceinline__ uint32_t slot_key(const Window &w, const CicGeom &g,
     int slot) {
   if (w.direct) return (uint32_t) slot;
-  const int uz = slot % g.ntz;
-  const int dy = (slot / g.ntz) % w.wy, dx = slot / (g.ntz * w.wy);
-  const int ux = (w.txlo + dx) % g.ntx, uy = (w.tylo + dy) % g.nty;
-  return (uint32_t) ((ux * g.nty + uy) * g.ntz + uz);
+  uint32_t uz;
+  const uint32_t q = fastdiv_qr((uint32_t) slot, g.fntz, uz);
+  const int dy = (int) (q % (uint32_t) w.wy), dx = (int) (q / (uint32_t) w.wy);
+  int ux = w.txlo + dx, uy = w.tylo + dy;
+  if (ux >= g.ntx) ux -= g.ntx;
+  if (uy >= g.nty) uy -= g.nty;
+  return tile_key(g, ux, uy, (int) uz);
+}
+
+// Add 1 per active lane to wcnt[slot] with one LDS atomic for the lanes that
+// share the first active lane's slot (the common case: a wave's particles
+// fall in the same tile) and one per lane otherwise.  RET: return each
+// lane's previous count (its rank).
+template <bool RET>
+__device__ __forceinline__ uint32_t wave_count_add(uint32_t *wcnt, bool active,
+    int slot) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t act = __ballot(active);
+  if (!act) return 0;
+  const int leader = __ffsll((unsigned long long) act) - 1;
+  const int lslot = __shfl(slot, leader, 64);
+  const uint64_t peers = __ballot(active && slot == lslot);
+  uint32_t base = 0;
+  if (lane == leader) {
+    if (RET) base = atomicAdd(&wcnt[lslot], (uint32_t) __popcll(peers));
+    else atomicAdd(&wcnt[lslot], (uint32_t) __popcll(peers));
+  }
+  if ((peers >> lane) & 1) {
+    if (!RET) return 0;
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t) __popcll(peers & ((1ull << lane) - 1));
+  }
+  if (!active) return 0;
+  if (RET) return atomicAdd(&wcnt[slot], 1u);
+  atomicAdd(&wcnt[slot], 1u);
+  return 0;
+}
+
+// The particle's tile span (n = 0 for indices past N).
+__device__ __forceinline__ int particle_span(const real *q0, const real *q1,
+    const real *q2, int u, uint32_t p, const CicGeom &g, TileSpan &s) {
+  if ((size_t) p >= g.N) return 0;
+  const ParticleCell c = cell_from(q0[u], q1[u], q2[u], p, g);
+  s = tile_span(c, g);
+  return s.ex * s.ey * s.ez;
 }
 
 template <int SCATTER>
@@ -161,7 +198,7 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
     unsigned long long *__restrict__ cursor, uint32_t *__restrict__ list) {
   __shared__ uint32_t wcnt[WMAX];
   __shared__ unsigned long long wbase[SCATTER ? WMAX : 1];
-  const size_t b0 = (size_t) blockIdx.x * CIC_PPB;
+  const uint32_t b0 = blockIdx.x * CIC_PPB;
   const Window w = make_window(g, b0);
   const int nslot = w.direct ? (int) ((size_t) g.ntx * g.nty * g.ntz)
                              : w.wx * w.wy * g.ntz;
@@ -172,44 +209,36 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
   real q[3][CIC_PPT];
 #pragma unroll
   for (int u = 0; u < CIC_PPT; u++) {
-    const size_t p = b0 + (size_t) u * CIC_TPB + threadIdx.x;
+    const size_t p = (size_t) b0 + (size_t) u * CIC_TPB + threadIdx.x;
     const bool ok = p < g.N;
     q[0][u] = ok ? p0[p] : (real) 0;
     q[1][u] = ok ? p1[p] : (real) 0;
     q[2][u] = ok ? p2[p] : (real) 0;
   }
-  uint32_t keys[8];
-  if (!SCATTER) {
+  // pass 1: count the window's tiles
 #pragma unroll
-    for (int u = 0; u < CIC_PPT; u++) {
-      const size_t p = b0 + (size_t) u * CIC_TPB + threadIdx.x;
-      if (p >= g.N) continue;
-      const ParticleCell c = cell_from(q[0][u], q[1][u], q[2][u], p, g.Ng, g.igs);
-      const int n = tiles_of(c, g, keys);
-      for (int v = 0; v < n; v++) {
-        const int s = window_slot(w, g, keys[v]);
-        if (s >= 0) atomicAdd(&wcnt[s], 1u);
-        else atomicAdd(&counts[keys[v]], 1u);
+  for (int u = 0; u < CIC_PPT; u++) {
+    const uint32_t p = b0 + (uint32_t) (u * CIC_TPB) + threadIdx.x;
+    TileSpan sp;
+    const int n = particle_span(q[0], q[1], q[2], u, p, g, sp);
+    for (int v = 0; __ballot(v < n); v++) {
+      int s = -1;
+      if (v < n) {
+        int ux, uy, uz;
+        span_tile(sp, g, v, ux, uy, uz);
+        s = window_slot(w, g, ux, uy, uz);
+        if (!SCATTER && s < 0) atomicAdd(&counts[tile_key(g, ux, uy, uz)], 1u);
       }
+      (void) wave_count_add<false>(wcnt, s >= 0, s);
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (!SCATTER) {
     for (int s = threadIdx.x; s < nslot; s += CIC_TPB)
       if (wcnt[s]) atomicAdd(&counts[slot_key(w, g, s)], wcnt[s]);
     return;
   }
-  // scatter: count the window, reserve each tile's range, place the indices
-#pragma unroll
-  for (int u = 0; u < CIC_PPT; u++) {
-    const size_t p = b0 + (size_t) u * CIC_TPB + threadIdx.x;
-    if (p >= g.N) continue;
-    const ParticleCell c = cell_from(q[0][u], q[1][u], q[2][u], p, g.Ng, g.igs);
-    const int n = tiles_of(c, g, keys);
-    for (int v = 0; v < n; v++) {
-      const int s = window_slot(w, g, keys[v]);
-      if (s >= 0) atomicAdd(&wcnt[s], 1u);
-    }
-  }
-  __syncthreads();
+  // scatter: reserve each window tile's range, then place the indices
   for (int s = threadIdx.x; s < nslot; s += CIC_TPB) {
     if (wcnt[s])
       wbase[s] = atomicAdd(&cursor[slot_key(w, g, s)], (unsigned long long) wcnt[s]);
@@ -218,27 +247,37 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < CIC_PPT; u++) {
-    const size_t p = b0 + (size_t) u * CIC_TPB + threadIdx.x;
-    if (p >= g.N) continue;
-    const ParticleCell c = cell_from(q[0][u], q[1][u], q[2][u], p, g.Ng, g.igs);
-    const int n = tiles_of(c, g, keys);
-    for (int v = 0; v < n; v++) {
-      const int s = window_slot(w, g, keys[v]);
-      unsigned long long pos;
-      if (s >= 0) pos = wbase[s] + atomicAdd(&wcnt[s], 1u);
-      else pos = atomicAdd(&cursor[keys[v]], 1ull);
-      list[pos] = (uint32_t) p;
+    const uint32_t p = b0 + (uint32_t) (u * CIC_TPB) + threadIdx.x;
+    TileSpan sp;
+    const int n = particle_span(q[0], q[1], q[2], u, p, g, sp);
+    for (int v = 0; __ballot(v < n); v++) {
+      int s = -1;
+      int ux = 0, uy = 0, uz = 0;
+      if (v < n) {
+        span_tile(sp, g, v, ux, uy, uz);
+        s = window_slot(w, g, ux, uy, uz);
+      }
+      const uint32_t r = wave_count_add<true>(wcnt, s >= 0, s);
+      if (v < n) {
+        unsigned long long pos;
+        if (s >= 0) pos = wbase[s] + r;
+        else pos = atomicAdd(&cursor[tile_key(g, ux, uy, uz)], 1ull);
+        list[pos] = p;
+      }
     }
   }
 }
 
-// One workgroup per tile.  MOM: also the three momentum fields.
+// One workgroup per tile.  MOM: also the three momentum fields, stored
+// normalised as the velocity gather uses them (pop_tracer.c:676-682): per
+// cell {rho_c, u0, u1, u2} with u_a = m_a / rho_c where rho_c > REAL_TOL
+// (the same IEEE division the gather would do per corner).
 template <int MOM, int TXX, int TYY, int TZZ>
 __global__ void __launch_bounds__(CIC_TPB)
 k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
     const real *__restrict__ p2, CicGeom g, const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ list, real *__restrict__ rho,
-    real *__restrict__ m0, real *__restrict__ m1, real *__restrict__ m2) {
+    const real *__restrict__ rho_norm, real4 *__restrict__ vel) {
   constexpr int TC = TXX * TYY * TZZ;
   constexpr int NF = MOM ? 4 : 1;
   // double accumulators: LDS double atomics are several times faster than
@@ -272,7 +311,7 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
   for (int u = 0; u < U; u++) {
     if (pv[u] == (size_t) -1) continue;
     const size_t p = pv[u];
-    const ParticleCell c = cell_from(qq[u][0], qq[u][1], qq[u][2], p, Ng, g.igs);
+    const ParticleCell c = cell_from(qq[u][0], qq[u][1], qq[u][2], (uint32_t) p, g);
     const real wx1 = c.x - (real) c.x0, wx0 = 1 - wx1;
     const real wy1 = c.y - (real) c.y0, wy0 = 1 - wy1;
     const real wz1 = c.z - (real) c.z0, wz0 = 1 - wz1;
@@ -318,9 +357,17 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
   for (int c = threadIdx.x; c < TC; c += CIC_TPB) {
     const int lz = c % TZZ, ly = (c / TZZ) % TYY, lx = c / (TZZ * TYY);
     const size_t gi = ((size_t) (ox + lx) * Ng + (oy + ly)) * Ng + (oz + lz);
-    if (rho) rho[gi] = (real) acc[0][c];
+    const real r = (real) acc[0][c];
+    if (rho) rho[gi] = r;
     if (MOM) {
-      m0[gi] = (real) acc[1][c]; m1[gi] = (real) acc[2][c]; m2[gi] = (real) acc[3][c];
+      const real rc = rho ? r : rho_norm[gi];
+      real u[3];
+#pragma unroll
+      for (int f = 0; f < 3; f++) {
+        const real mv = (real) acc[1 + f][c];
+        u[f] = ((double) rc > EZ_REAL_TOL) ? mv / rc : mv;
+      }
+      vel[gi] = real4{rc, u[0], u[1], u[2]};
     }
   }
 }
@@ -346,8 +393,11 @@ CicGeom cic_geometry(int Ng, real igs) {
   g.TX = tile_dim(Ng, 8);
   g.TY = tile_dim(Ng, 8);
   g.TZ = tile_dim(Ng, 32);
+  g.lx = __builtin_ctz(g.TX); g.ly = __builtin_ctz(g.TY); g.lz = __builtin_ctz(g.TZ);
   g.ntx = Ng / g.TX; g.nty = Ng / g.TY; g.ntz = Ng / g.TZ;
   g.N = (size_t) Ng * Ng * Ng;
+  g.fNg = fastdiv_make((uint32_t) Ng);
+  g.fntz = fastdiv_make((uint32_t) g.ntz);
   return g;
 }
 
@@ -356,24 +406,24 @@ size_t cic_nkeys(const CicGeom &g) { return (size_t) g.ntx * g.nty * g.ntz; }
 template <int MOM, int A, int B, int C>
 static void launch_tile(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    real *m0, real *m1, real *m2, hipStream_t st) {
+    const real *rho_norm, real4 *vel, hipStream_t st) {
   const unsigned nt = (unsigned) (g.ntx * g.nty * g.ntz);
   hipLaunchKernelGGL((k_cic_tile<MOM, A, B, C>), dim3(nt), dim3(CIC_TPB), 0, st,
-      p0, p1, p2, g, off, list, rho, m0, m1, m2);
+      p0, p1, p2, g, off, list, rho, rho_norm, vel);
 }
 
 template <int MOM>
 static hipError_t dispatch_tile(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    real *m0, real *m1, real *m2, hipStream_t st) {
+    const real *rho_norm, real4 *vel, hipStream_t st) {
   if (g.TX == 8 && g.TY == 8 && g.TZ == 32)
-    launch_tile<MOM, 8, 8, 32>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
+    launch_tile<MOM, 8, 8, 32>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
   else if (g.TX == 8 && g.TY == 8 && g.TZ == 16)
-    launch_tile<MOM, 8, 8, 16>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
+    launch_tile<MOM, 8, 8, 16>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
   else if (g.TX == 8 && g.TY == 8 && g.TZ == 8)
-    launch_tile<MOM, 8, 8, 8>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
+    launch_tile<MOM, 8, 8, 8>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
   else if (g.TX == 4 && g.TY == 4 && g.TZ == 4)
-    launch_tile<MOM, 4, 4, 4>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
+    launch_tile<MOM, 4, 4, 4>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -410,11 +460,12 @@ hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
   return hipGetLastError();
 }
 
-// Tile accumulation: density into rho (unless null) and, when m0 is not
-// null, the three momentum fields.
+// Tile accumulation: density into rho (unless null) and, when vel is not
+// null, the normalised momentum fields.
 hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    real *m0, real *m1, real *m2, hipStream_t st) {
-  if (m0) return dispatch_tile<1>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
-  return dispatch_tile<0>(p0, p1, p2, g, off, list, rho, m0, m1, m2, st);
+    const real *rho_norm, real4 *vel, hipStream_t st) {
+  if (vel && !rho && !rho_norm) return hipErrorInvalidValue;
+  if (vel) return dispatch_tile<1>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+  return dispatch_tile<0>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
 }

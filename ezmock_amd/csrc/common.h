@@ -42,11 +42,13 @@ enum {
 #define EZ_REAL_MIN 1.17549435082228750797e-38F
 #define EZ_REAL_TOL 1e-5
 typedef float2 cplx;
+typedef float4 real4;
 typedef uint32_t rkey;          // order-preserving key of a positive real
 #else
 #define EZ_REAL_MIN 2.2250738585072013830902e-308
 #define EZ_REAL_TOL 1e-10
 typedef double2 cplx;
+typedef double4 real4;
 typedef uint64_t rkey;
 #endif
 
@@ -70,4 +72,39 @@ typedef uint64_t rkey;
 
 static inline unsigned ez_blocks(size_t n, unsigned per) {
   return (unsigned) ((n + per - 1) / per);
+}
+
+// Division of 32-bit unsigned integers by a run-time invariant divisor d
+// (1 <= d < 2^32) with one high multiply, exact for every numerator
+// (T. Granlund and P. L. Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, fig. 4.1): l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n),
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0).
+struct FastDiv {
+  uint32_t d, m;
+  int s1, s2;
+};
+
+static inline FastDiv fastdiv_make(uint32_t d) {
+  FastDiv f;
+  int l = 0;
+  while (l < 32 && (1ull << l) < d) l++;
+  f.d = d;
+  f.m = (uint32_t) ((((unsigned __int128) 1 << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 1 ? l - 1 : 0;
+  return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t fastdiv_q(uint32_t n, const FastDiv &f) {
+  const uint32_t t = (uint32_t) (((uint64_t) f.m * n) >> 32);   // mul_hi
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// n = q d + r
+__host__ __device__ __forceinline__ uint32_t fastdiv_qr(uint32_t n, const FastDiv &f,
+    uint32_t &r) {
+  const uint32_t q = fastdiv_q(n, f);
+  r = n - q * f.d;
+  return q;
 }

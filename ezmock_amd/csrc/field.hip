@@ -51,90 +51,116 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 
 // ------------------------------------------------------------ spectrum
 // One thread = CH consecutive cells of the half-complex grid (reference
-// order, k fastest, Ngk = Ng/2 + 1 per row).  Cell idx >= 1 consumes draws
-// (idx-1)*2 (amplitude, then phase) or (idx-1) with fixed amplitudes.
-// Output: rhok[i][j][kk < Ng/2] and rhok_ny[i][j] (the kz = Ng/2 plane);
-// optional copy in the reference layout.
+// order, k fastest, Ngk = Ng/2 + 1 per row; Ng even).  Cell idx >= 1
+// consumes draws (idx-1)*2 (amplitude, then phase) or (idx-1) with fixed
+// amplitudes.  Output: rhok[i][j][kk < Ng/2] and rhok_ny[i][j] (the
+// kz = Ng/2 plane); optional copy in the reference layout.
+// The block's 256 CH cells map to one contiguous range of rhok (minus the
+// Nyquist cells): values are staged in LDS and stored by consecutive lanes
+// to consecutive addresses.  Mirror writes of the kz = 0 and Nyquist planes
+// (2 cells in Ngk) go straight to memory.
+#define GEN_TPB 256
+#define GEN_SKIP 0xffffu
+
 template <int CH>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(GEN_TPB)
 k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
     real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
     const uint32_t *__restrict__ jtab) {
-  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t idx0 = t * CH;
-  if (idx0 >= gp.Nh) return;
+  constexpr int NB = GEN_TPB * CH;
+  __shared__ cplx sv[NB + NB / CH];             // one pad slot per thread
+  __shared__ uint16_t so[NB];
   const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
-  const size_t plane = (size_t) Ng * Ngk;
+  const size_t blk0 = (size_t) blockIdx.x * NB;
+  // rhok index of a cell k < Ngh: row * Ngh + k = idx - row
+  const size_t obase = blk0 - blk0 / (size_t) Ngk;
+  const size_t idx0 = blk0 + (size_t) threadIdx.x * CH;
+  size_t row = idx0 / (size_t) Ngk;
+  int k = (int) (idx0 - row * (size_t) Ngk);
+  int i = (int) (row / (size_t) Ng);
+  int j = (int) (row - (size_t) i * Ng);
   MrgState g = gp.seed_state;
-  const uint64_t mult = gp.fixamp ? 1 : 2;
-  mrg_jump(g, (idx0 == 0 ? 0 : (idx0 - 1)) * mult, jtab);
+  if (idx0 < gp.Nh) {
+    const uint64_t mult = gp.fixamp ? 1 : 2;
+    mrg_jump(g, (idx0 == 0 ? 0 : (idx0 - 1)) * mult, jtab);
+  }
   for (int c = 0; c < CH; c++) {
     const size_t idx = idx0 + c;
-    if (idx >= gp.Nh) break;
-    const int i = (int) (idx / plane);
-    const size_t rem = idx - (size_t) i * plane;
-    const int j = (int) (rem / Ngk);
-    const int k = (int) (rem - (size_t) j * Ngk);
-    real re = 0, im = 0;
-    bool write = true, mirror = false;
-    int ni = 0, nj = 0;
-    if (idx != 0) {
-      const double amp = gp.fixamp ? 1.0 : mrg_next(g);
-      double ph = mrg_next(g) * 2 * EZ_PI;
-      const bool plane_k = (k == 0) || (k == Ngh && (Ng & 1) == 0);
-      if (plane_k) {
-        ni = (i == 0) ? 0 : Ng - i;
-        nj = (j == 0) ? 0 : Ng - j;
-        const bool lower = (i == ni && j <= Ngh) || (i != ni && i <= Ngh);
-        if (!lower) write = false;      // filled by its mirror
-        mirror = lower;
+    const int e = threadIdx.x * CH + c;
+    uint16_t off = GEN_SKIP;
+    if (idx < gp.Nh) {
+      real re = 0, im = 0;
+      bool write = true, mirror = false;
+      int ni = 0, nj = 0;
+      if (idx != 0) {
+        const double amp = gp.fixamp ? 1.0 : mrg_next(g);
+        double ph = mrg_next(g) * 2 * EZ_PI;
+        const bool plane_k = (k == 0) || (k == Ngh);
+        if (plane_k) {
+          ni = (i == 0) ? 0 : Ng - i;
+          nj = (j == 0) ? 0 : Ng - j;
+          const bool lower = (i == ni && j <= Ngh) || (i != ni && i <= Ngh);
+          if (!lower) write = false;    // filled by its mirror
+          mirror = lower;
+        }
+        if (write) {
+          const int ki = (i <= Ngh) ? i : i - Ng;
+          const int kj = (j <= Ngh) ? j : j - Ng;
+          const int ksqi = ki * ki + kj * kj + k * k;
+          const double ksq = (double) ksqi;
+          double P = ptab[ksqi];
+          double a = gp.fixamp ? 1.0 : -log(amp);
+          if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
+          else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
+          P *= gp.Pfac;
+          if (gp.iphase) ph += EZ_PI;
+          P /= (ksq * gp.kfac);
+          re = (real) (-P * sin(ph));
+          im = (real) (P * cos(ph));
+          // perturb.c:341-359: self-conjugate modes keep only the imaginary
+          // part (the reference writes (+P sin, P cos) twice, then zeroes re).
+          if (mirror && i == ni && j == nj) re = 0;
+        }
       }
       if (write) {
-        const int ki = (i <= Ngh) ? i : i - Ng;
-        const int kj = (j <= Ngh) ? j : j - Ng;
-        const int ksqi = ki * ki + kj * kj + k * k;
-        const double ksq = (double) ksqi;
-        double P = ptab[ksqi];
-        double a = gp.fixamp ? 1.0 : -log(amp);
-        if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
-        else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
-        P *= gp.Pfac;
-        if (gp.iphase) ph += EZ_PI;
-        P /= (ksq * gp.kfac);
-        re = (real) (-P * sin(ph));
-        im = (real) (P * cos(ph));
-        // perturb.c:341-359: self-conjugate modes keep only the imaginary
-        // part (the reference writes (+P sin, P cos) twice, then zeroes re).
-        if (mirror && i == ni && j == nj) re = 0;
-      }
-    }
-    else {
-      write = true;                     // (0,0,0) = 0
-    }
-    if (write) {
-      const cplx v = {re, im};
-      if (k < Ngh || (Ng & 1)) rhok[((size_t) i * Ng + j) * Ngh + k] = v;
-      else rhok_ny[(size_t) i * Ng + j] = v;
-      if (ref_copy) { ref_copy[2 * idx] = re; ref_copy[2 * idx + 1] = im; }
-      if (mirror && !(i == ni && j == nj)) {
-        const cplx w = {-re, im};
-        if (k < Ngh) rhok[((size_t) ni * Ng + nj) * Ngh + k] = w;
-        else rhok_ny[(size_t) ni * Ng + nj] = w;
-        if (ref_copy) {
-          const size_t nidx = ((size_t) ni * Ng + nj) * Ngk + k;
-          ref_copy[2 * nidx] = -re; ref_copy[2 * nidx + 1] = im;
+        const cplx v = {re, im};
+        if (k < Ngh) {
+          sv[e + e / CH] = v;
+          off = (uint16_t) (idx - row - obase);
+        }
+        else rhok_ny[row] = v;
+        if (ref_copy) { ref_copy[2 * idx] = re; ref_copy[2 * idx + 1] = im; }
+        if (mirror && !(i == ni && j == nj)) {
+          const cplx w = {-re, im};
+          if (k < Ngh) rhok[((size_t) ni * Ng + nj) * Ngh + k] = w;
+          else rhok_ny[(size_t) ni * Ng + nj] = w;
+          if (ref_copy) {
+            const size_t nidx = ((size_t) ni * Ng + nj) * Ngk + k;
+            ref_copy[2 * nidx] = -re; ref_copy[2 * nidx + 1] = im;
+          }
         }
       }
     }
+    so[e] = off;
+    if (++k == Ngk) {
+      k = 0; row++;
+      if (++j == Ng) { j = 0; i++; }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB; e += GEN_TPB) {
+    const uint16_t o = so[e];
+    if (o != GEN_SKIP) rhok[obase + o] = sv[e + e / CH];
   }
 }
 
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
     hipStream_t st) {
-  constexpr int CH = 16;
+  if (gp.Ng & 1) return hipErrorInvalidValue;
+  constexpr int CH = sizeof(real) == 4 ? 16 : 8;
   const size_t nthr = (gp.Nh + CH - 1) / CH;
-  hipLaunchKernelGGL(k_gen_rhok<CH>, dim3(ez_blocks(nthr, 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_gen_rhok<CH>, dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
       st, rhok, rhok_ny, ref_copy, gp, ptab, jtab);
   return hipGetLastError();
 }

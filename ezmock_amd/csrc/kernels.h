@@ -29,9 +29,11 @@ hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st);
 struct CicGeom {
   int Ng;
   real igs;
-  int TX, TY, TZ;               // tile shape in cells
+  int TX, TY, TZ;               // tile shape in cells (powers of two)
+  int lx, ly, lz;               // their base-2 logarithms
   int ntx, nty, ntz;            // tiles per dimension
-  size_t N;
+  size_t N;                     // particles (< 2^32)
+  FastDiv fNg, fntz;            // division by Ng and by ntz
 };
 CicGeom cic_geometry(int Ng, real igs);
 size_t cic_nkeys(const CicGeom &g);
@@ -40,9 +42,12 @@ hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
     uint64_t *scan_tmp, hipStream_t st);
 hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, uint64_t *cursor, uint32_t *list, hipStream_t st);
+// vel (optional): per cell {rho_c, u0, u1, u2}, u_a = m_a / rho_c where
+// rho_c > REAL_TOL and m_a otherwise (m_a the painted momentum); rho_c is
+// the density painted in the same pass, or rho_norm when rho is null.
 hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    real *m0, real *m1, real *m2, hipStream_t st);
+    const real *rho_norm, real4 *vel, hipStream_t st);
 
 // tracer.hip
 struct BiasParams {
@@ -78,8 +83,7 @@ hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
     real *x, real *y, real *z, size_t ntot, hipStream_t st);
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
-    const real *rho, const real *m0, const real *m1, const real *m2,
-    size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
     hipStream_t st);
 hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,

@@ -163,6 +163,7 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     const uint64_t *__restrict__ blockoff, unsigned long long *__restrict__ hist,
     BiasParams bp, const uint32_t *__restrict__ jtab) {
   __shared__ real tile[PADIDX(CPB)];
+  __shared__ uint16_t vcell[CPB];       // block-local rank -> cell
   __shared__ uint32_t lh[EZ_NUM_LOGB_BIN];
   __shared__ uint64_t ws[TPB / 64];
   __shared__ MrgState base;
@@ -176,46 +177,62 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     tile[PADIDX(c)] = (i < bp.N) ? rho[i] : (real) 0;
   }
   __syncthreads();
-  // thread tid owns cells c = 16 tid .. 16 tid + 15 of the block
+  // thread tid owns cells c = 16 tid .. 16 tid + 15 of the block: rank the
+  // valid ones, clear the others
   uint64_t cnt = 0;
+  uint32_t vmask = 0;
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
     const int c = tid * CPT + q;
-    if (b0 + c < bp.N && tile[PADIDX(c)] >= bp.thres) cnt++;
+    if (b0 + c < bp.N && tile[PADIDX(c)] >= bp.thres) { cnt++; vmask |= 1u << q; }
   }
   uint64_t tot;
   const uint64_t excl = block_excl_scan<uint64_t>(cnt, tot, ws);
+  {
+    int l = (int) excl;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int c = tid * CPT + q;
+      if ((vmask >> q) & 1) vcell[l++] = (uint16_t) c;
+      else tile[PADIDX(c)] = 0;
+    }
+  }
   const uint64_t rb = blockoff[blockIdx.x];
   const uint64_t pb = rb >> 1;
-  if (tid == 0) {
-    MrgState g = bp.seed_state;
-    mrg_jump(g, (uint64_t) 2 * bp.N + 2 * pb, jtab);
+  // pairs pb .. pe cover global ranks rb .. rb + tot - 1
+  const uint64_t npair = tot ? ((rb + tot - 1) >> 1) - pb + 1 : 0;
+  const uint64_t ppt = (npair + TPB - 1) / TPB;
+  if (tid == 0 && npair) {
+    MrgState g = bp.seed_state;            // advanced to draw 2N
+    mrg_jump(g, 2 * pb, jtab);
     base = g;
   }
   __syncthreads();
-  uint64_t r = rb + excl;
-  MrgState g = base;
-  if (cnt) mrg_jump(g, 2 * ((r >> 1) - pb), jtab);
-  double gg[2] = {0, 0};
-  bool have = false;
+  // thread tid draws pairs pb + tid ppt .. pb + (tid + 1) ppt - 1 in order:
+  // one Box-Muller pair per iteration on every lane
+  const uint64_t m0 = (uint64_t) tid * ppt;
+  if (m0 < npair) {
+    MrgState g = base;
+    mrg_jump(g, 2 * m0, jtab);
+    const uint64_t m1 = (m0 + ppt < npair) ? m0 + ppt : npair;
 #pragma unroll 1
-  for (int q = 0; q < CPT; q++) {
-    const int c = tid * CPT + q;
-    if (b0 + c >= bp.N) break;
-    const real v = tile[PADIDX(c)];
-    if (v < bp.thres) { tile[PADIDX(c)] = 0; continue; }
-    const real rt = (v > bp.rho_sat) ? bp.rho_sat : v;
-    if ((r & 1) == 0 || !have) {
+    for (uint64_t m = m0; m < m1; m++) {
+      double gg[2];
       gauss_pair(g, bp.lambda, gg[0], gg[1]);
-      gg[0] = (gg[0] >= 0) ? 1 + gg[0] : exp(gg[0]);
-      gg[1] = (gg[1] >= 0) ? 1 + gg[1] : exp(gg[1]);
-      have = true;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint64_t r = 2 * (pb + m) + h;          // global rank
+        if (r < rb || r >= rb + tot) continue;
+        const double gp = (gg[h] >= 0) ? 1 + gg[h] : exp(gg[h]);
+        const int c = vcell[r - rb];
+        const real v = tile[PADIDX(c)];
+        const real rt = (v > bp.rho_sat) ? bp.rho_sat : v;
+        const double tmp = (1 - exp((double) rt * bp.iexp)) * gp;
+        const real o = (real) tmp;
+        tile[PADIDX(c)] = o;
+        atomicAdd(&lh[hist_bin(o, bp.imax)], 1u);
+      }
     }
-    const double tmp = (1 - exp((double) rt * bp.iexp)) * gg[r & 1];
-    const real o = (real) tmp;
-    tile[PADIDX(c)] = o;
-    atomicAdd(&lh[hist_bin(o, bp.imax)], 1u);
-    r++;
   }
   __syncthreads();
 #pragma unroll 4
@@ -231,8 +248,10 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
     unsigned long long *hist, const BiasParams &bp, const uint32_t *jtab,
     hipStream_t st) {
+  BiasParams b2 = bp;
+  mrg_jump(b2.seed_state, (uint64_t) 2 * bp.N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_bias, dim3(ez_blocks(bp.N, CPB)), dim3(TPB), 0, st,
-      rho, rhot, blockoff, hist, bp, jtab);
+      rho, rhot, blockoff, hist, b2, jtab);
   return hipGetLastError();
 }
 
@@ -469,12 +488,17 @@ hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
 }
 
 // ------------------------------------------------------------ positions
+// Per block of CPB cells: the tracer counts are ranked (block scan), then
+// each thread generates a contiguous run of the block's tracers, so every
+// lane draws one tracer per iteration whatever the occupancy of its cells.
+// seed: the stream already advanced to draw 5N.
 __global__ void __launch_bounds__(TPB)
 k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockofs,
-    int Ng, double Lbox, real gs, size_t N, MrgState seed,
+    FastDiv fNg, double Lbox, real gs, size_t N, MrgState seed,
     const uint32_t *__restrict__ jtab, real *__restrict__ x,
     real *__restrict__ y, real *__restrict__ z, size_t ntot) {
   __shared__ int16_t tile[PADIDX(CPB)];
+  __shared__ uint32_t toff[TPB];
   __shared__ uint64_t ws[TPB / 64];
   __shared__ MrgState base;
   const size_t b0 = (size_t) blockIdx.x * CPB;
@@ -491,39 +515,55 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
   for (int q = 0; q < CPT; q++) cnt += (uint64_t) tile[PADIDX(tid * CPT + q)];
   uint64_t tot;
   const uint64_t excl = block_excl_scan<uint64_t>(cnt, tot, ws);
-  if (tot == 0) return;
+  if (tot == 0) return;                 // uniform over the block
+  toff[tid] = (uint32_t) excl;          // tot <= CPB * 1024
   const uint64_t tb = blockofs[blockIdx.x];
   if (tid == 0) {
     MrgState g = seed;
-    mrg_jump(g, (uint64_t) 5 * N + 3 * tb, jtab);
+    mrg_jump(g, 3 * tb, jtab);
     base = g;
   }
   __syncthreads();
-  if (cnt == 0) return;
+  const uint32_t K = (uint32_t) ((tot + TPB - 1) / TPB);
+  uint32_t l = (uint32_t) tid * K;
+  if (l >= tot) return;
+  const uint32_t lend = (l + K < tot) ? l + K : (uint32_t) tot;
+  // cell of tracer l: the last thread range starting at or before l, then
+  // its cells
+  int lo = 0, hi = TPB - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= l) lo = mid; else hi = mid - 1;
+  }
+  int c = lo * CPT;
+  uint32_t acc = toff[lo];
+  int m = tile[PADIDX(c)];
+  while (acc + (uint32_t) m <= l) { acc += (uint32_t) m; c++; m = tile[PADIDX(c)]; }
+  int n = (int) (l - acc);
   MrgState g = base;
-  mrg_jump(g, 3 * excl, jtab);
-  size_t t = tb + excl;
+  mrg_jump(g, 3 * (uint64_t) l, jtab);
+  uint32_t k, j;
+  uint32_t ij = fastdiv_qr((uint32_t) (b0 + c), fNg, k);
+  uint32_t i = fastdiv_qr(ij, fNg, j);
 #pragma unroll 1
-  for (int q = 0; q < CPT; q++) {
-    const int c = tid * CPT + q;
-    const int m = tile[PADIDX(c)];
-    if (m <= 0) continue;
-    const size_t cell = b0 + c;
-    const int k = (int) (cell % Ng);
-    const size_t ij = cell / Ng;
-    const int j = (int) (ij % Ng), i = (int) (ij / Ng);
-    for (int n = 0; n < m; n++) {
-      double r = mrg_next(g) - 0.5;
-      real xx = (real) ((i + r) * (double) gs);
-      r = mrg_next(g) - 0.5;
-      real yy = (real) ((j + r) * (double) gs);
-      r = mrg_next(g) - 0.5;
-      real zz = (real) ((k + r) * (double) gs);
-      if (xx < 0) xx = (real) ((double) xx + Lbox);
-      if (yy < 0) yy = (real) ((double) yy + Lbox);
-      if (zz < 0) zz = (real) ((double) zz + Lbox);
-      if (t < ntot) { x[t] = xx; y[t] = yy; z[t] = zz; }
-      t++;
+  for (;;) {
+    double r = mrg_next(g) - 0.5;
+    real xx = (real) (((int) i + r) * (double) gs);
+    r = mrg_next(g) - 0.5;
+    real yy = (real) (((int) j + r) * (double) gs);
+    r = mrg_next(g) - 0.5;
+    real zz = (real) (((int) k + r) * (double) gs);
+    if (xx < 0) xx = (real) ((double) xx + Lbox);
+    if (yy < 0) yy = (real) ((double) yy + Lbox);
+    if (zz < 0) zz = (real) ((double) zz + Lbox);
+    const size_t t = tb + l;
+    if (t < ntot) { x[t] = xx; y[t] = yy; z[t] = zz; }
+    if (++l >= lend) break;
+    if (++n == m) {                     // next occupied cell of the block
+      n = 0;
+      do { c++; m = tile[PADIDX(c)]; } while (m <= 0);
+      ij = fastdiv_qr((uint32_t) (b0 + c), fNg, k);
+      i = fastdiv_qr(ij, fNg, j);
     }
   }
 }
@@ -532,8 +572,11 @@ hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
     real *x, real *y, real *z, size_t ntot, hipStream_t st) {
   const size_t N = (size_t) Ng * Ng * Ng;
+  if (N >= 0xffffffffull) return hipErrorInvalidValue;
+  MrgState s5 = seed;
+  mrg_jump(s5, (uint64_t) 5 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_positions, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      nt, blockofs, Ng, Lbox, gs, N, seed, jtab, x, y, z, ntot);
+      nt, blockofs, fastdiv_make((uint32_t) Ng), Lbox, gs, N, s5, jtab, x, y, z, ntot);
   return hipGetLastError();
 }
 
@@ -544,10 +587,8 @@ hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
 // Gaussian scatter of pair q from draws 8N + 6q.
 #define PPT 4
 
-__device__ __forceinline__ void gather_vel(const real *__restrict__ rho,
-    const real *__restrict__ m0, const real *__restrict__ m1,
-    const real *__restrict__ m2, int Ng, real igs, double vfac, real px,
-    real py, real pz, real *v) {
+__device__ __forceinline__ void gather_vel(const real4 *__restrict__ vel,
+    int Ng, real igs, double vfac, real px, real py, real pz, real *v) {
   const real xx = px * igs, yy = py * igs, zz = pz * igs;
   int x0 = (int) xx, y0 = (int) yy, z0 = (int) zz;
   const real wx1 = xx - (real) x0, wx0 = 1 - wx1;
@@ -567,35 +608,32 @@ __device__ __forceinline__ void gather_vel(const real *__restrict__ rho,
     ((size_t) x0 * Ng + y1) * Ng + z0, ((size_t) x0 * Ng + y1) * Ng + z1,
     ((size_t) x1 * Ng + y0) * Ng + z0, ((size_t) x1 * Ng + y0) * Ng + z1,
     ((size_t) x1 * Ng + y1) * Ng + z0, ((size_t) x1 * Ng + y1) * Ng + z1};
-  real rc[8], r[8];
+  // per corner {rho, u0, u1, u2}: u already normalised by rho (disp_cic)
+  real4 f[8];
 #pragma unroll
-  for (int q = 0; q < 8; q++) rc[q] = rho[c[q]];
+  for (int q = 0; q < 8; q++) f[q] = vel[c[q]];
+  real r[8];
   // r_q = rho * wx * wy * wz, evaluated left to right as assign_vel does
-  r[0] = rc[0] * wx0 * wy0 * wz0; r[1] = rc[1] * wx0 * wy0 * wz1;
-  r[2] = rc[2] * wx0 * wy1 * wz0; r[3] = rc[3] * wx0 * wy1 * wz1;
-  r[4] = rc[4] * wx1 * wy0 * wz0; r[5] = rc[5] * wx1 * wy0 * wz1;
-  r[6] = rc[6] * wx1 * wy1 * wz0; r[7] = rc[7] * wx1 * wy1 * wz1;
+  r[0] = f[0].x * wx0 * wy0 * wz0; r[1] = f[1].x * wx0 * wy0 * wz1;
+  r[2] = f[2].x * wx0 * wy1 * wz0; r[3] = f[3].x * wx0 * wy1 * wz1;
+  r[4] = f[4].x * wx1 * wy0 * wz0; r[5] = f[5].x * wx1 * wy0 * wz1;
+  r[6] = f[6].x * wx1 * wy1 * wz0; r[7] = f[7].x * wx1 * wy1 * wz1;
   const real rs = r[0] + r[1] + r[2] + r[3] + r[4] + r[5] + r[6] + r[7];
-  const real *ms[3] = {m0, m1, m2};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    real a[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const real mv = ms[d][c[q]];
-      a[q] = ((double) rc[q] > EZ_REAL_TOL) ? mv / rc[q] : mv;
-    }
-    real s = a[0] * r[0] + a[1] * r[1] + a[2] * r[2] + a[3] * r[3]
-           + a[4] * r[4] + a[5] * r[5] + a[6] * r[6] + a[7] * r[7];
-    v[d] = (real) ((double) s * (vfac / (double) rs));
-  }
+  real sx = f[0].y * r[0] + f[1].y * r[1] + f[2].y * r[2] + f[3].y * r[3]
+          + f[4].y * r[4] + f[5].y * r[5] + f[6].y * r[6] + f[7].y * r[7];
+  real sy = f[0].z * r[0] + f[1].z * r[1] + f[2].z * r[2] + f[3].z * r[3]
+          + f[4].z * r[4] + f[5].z * r[5] + f[6].z * r[6] + f[7].z * r[7];
+  real sz = f[0].w * r[0] + f[1].w * r[1] + f[2].w * r[2] + f[3].w * r[3]
+          + f[4].w * r[4] + f[5].w * r[5] + f[6].w * r[6] + f[7].w * r[7];
+  const double sc = vfac / (double) rs;
+  v[0] = (real) ((double) sx * sc);
+  v[1] = (real) ((double) sy * sc);
+  v[2] = (real) ((double) sz * sc);
 }
 
 __global__ void __launch_bounds__(TPB)
 k_velocities(const real *__restrict__ x, const real *__restrict__ y,
-    const real *__restrict__ z, const real *__restrict__ rho,
-    const real *__restrict__ m0, const real *__restrict__ m1,
-    const real *__restrict__ m2, size_t ntot, int Ng, real igs, double vfac,
+    const real *__restrict__ z, const real4 *__restrict__ vel, size_t ntot, int Ng, real igs, double vfac,
     double sigma, uint64_t base_draw, MrgState seed,
     const uint32_t *__restrict__ jtab, real *__restrict__ vx,
     real *__restrict__ vy, real *__restrict__ vz) {
@@ -618,7 +656,7 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     if (q >= npair) break;
     const size_t a = 2 * q, b = a + 1;
     real va[3], vb3[3];
-    gather_vel(rho, m0, m1, m2, Ng, igs, vfac, x[a], y[a], z[a], va);
+    gather_vel(vel, Ng, igs, vfac, x[a], y[a], z[a], va);
     double g1, g2;
     gauss_pair(g, sigma, g1, g2);
     va[0] = (real) ((double) va[0] + g1);
@@ -627,7 +665,7 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     va[2] = (real) ((double) va[2] + g1);
     vx[a] = va[0]; vy[a] = va[1]; vz[a] = va[2];
     if (b < ntot) {                       // else: the last, unpaired tracer
-      gather_vel(rho, m0, m1, m2, Ng, igs, vfac, x[b], y[b], z[b], vb3);
+      gather_vel(vel, Ng, igs, vfac, x[b], y[b], z[b], vb3);
       vb3[0] = (real) ((double) vb3[0] + g2);
       gauss_pair(g, sigma, g1, g2);
       vb3[1] = (real) ((double) vb3[1] + g1);
@@ -638,16 +676,17 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
 }
 
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
-    const real *rho, const real *m0, const real *m1, const real *m2,
-    size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
     hipStream_t st) {
   const size_t N = (size_t) Ng * Ng * Ng;
   const size_t npair = (ntot + 1) / 2;
   if (npair == 0) return hipSuccess;
+  MrgState s8 = seed;                       // draw 8N
+  mrg_jump(s8, (uint64_t) 8 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_velocities, dim3(ez_blocks(npair, TPB * PPT)), dim3(TPB),
-      0, st, x, y, z, rho, m0, m1, m2, ntot, Ng, igs, vfac, sigma_v,
-      (uint64_t) 8 * N, seed, jtab, vx, vy, vz);
+      0, st, x, y, z, vel, ntot, Ng, igs, vfac, sigma_v,
+      (uint64_t) 0, s8, jtab, vx, vy, vz);
   return hipGetLastError();
 }
 
