@@ -97,6 +97,74 @@ def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=T
     return dt, ntr, kt, lat
 
 
+def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
+    """Calibration sweep: nmock mocks at Ng^3 with seeds 1..nmock and bias
+    parameters drawn from a fixed meta-seed (rho_c in [0.2,1.2], rho_exp in
+    [5,20], pdf_base in [0.1,0.5], sigma_v in [0,300]), 3e5 tracers each,
+    through the request pool (all submitted at once)."""
+    import numpy as np
+    from ezmock_amd import inputs
+    from ezmock_amd.pool import MockPool
+    k, P = inputs.eh_nowiggle_pk()
+    rs = np.random.default_rng(2024)
+    par = [(rs.uniform(0.2, 1.2), rs.uniform(5, 20), rs.uniform(0.1, 0.5), rs.uniform(0, 300))
+           for _ in range(nmock)]
+    pool = MockPool(k, P, 0.5, 0.31, nslot=nslot, mem_budget=200e9)
+    for i in range(2):                                      # warm the workers
+        pool.wait(pool.submit(Ng, L, 1000 + i, par[i], 300000))
+    t0 = time.perf_counter()
+    ids = [pool.submit(Ng, L, i + 1, par[i], 300000) for i in range(nmock)]
+    res = [pool.wait(i) for i in ids]
+    dt = time.perf_counter() - t0
+    pool.close()
+    bad = [r["errmsg"] for r in res if r["err"]]
+    return {"mocks": nmock, "grid": Ng, "slots": nslot, "seconds": round(dt, 3),
+            "mocks_per_s": round(nmock / dt, 2), "errors": bad[:3]}
+
+
+def run_serving(nreq=400, rate=20.0, nslot=4, budget=200e9, seed=7):
+    """Serving loop: nreq requests arriving Poisson at `rate` per second,
+    grids {128,192,256,384,512}^3 with weights {.3,.25,.2,.15,.1},
+    Lbox = 3.9 Ng, bias parameters uniform in the calibration ranges,
+    3e-4 (h/Mpc)^3 tracers, admitted by a device-memory budget."""
+    import numpy as np
+    from ezmock_amd import inputs
+    from ezmock_amd.pool import MockPool
+    k, P = inputs.eh_nowiggle_pk()
+    rs = np.random.default_rng(seed)
+    grids = rs.choice([128, 192, 256, 384, 512], size=nreq, p=[.3, .25, .2, .15, .1])
+    gaps = rs.exponential(1.0 / rate, size=nreq)
+    pool = MockPool(k, P, 0.5, 0.31, nslot=nslot, mem_budget=budget)
+    for Ng in (128, 512):                                    # warm the workers
+        pool.wait(pool.submit(Ng, 3.9 * Ng, 99, (0.5, 10, 0.3, 150), int(3e-4 * (3.9 * Ng) ** 3)))
+    ids = []
+    t0 = time.perf_counter()
+    tnext = t0
+    for i in range(nreq):
+        tnext += gaps[i]
+        d = tnext - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        Ng = int(grids[i])
+        L = 3.9 * Ng
+        ids.append(pool.submit(Ng, L, 1 + i, (rs.uniform(0.2, 1.2), rs.uniform(5, 20),
+                                              rs.uniform(0.1, 0.5), rs.uniform(0, 300)),
+                               int(round(3e-4 * L ** 3))))
+        if (i + 1) % 100 == 0:
+            log(f"  serving: {i + 1}/{nreq} submitted")
+    res = [pool.wait(i) for i in ids]
+    dt = time.perf_counter() - t0
+    pool.close()
+    lat = np.array([r["latency"] for r in res]) * 1e3
+    ok = [r for r in res if not r["err"]]
+    return {"requests": nreq, "rate_per_s": rate, "slots": nslot,
+            "errors": len(res) - len(ok), "seconds": round(dt, 2),
+            "mocks_per_s": round(len(ok) / dt, 2),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "latency_ms_max": round(float(lat.max()), 2)}
+
+
 def cpu_baseline(precision):
     """The oracle on the host cores, one mock of the same workload."""
     from oracle import pyoracle as po
@@ -128,7 +196,9 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the secondary 1024^3 FP32 and 512^3 FP64 lines")
+                    help="skip the secondary lines (1024^3 FP32, 512^3 FP64, batch, serving)")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests of the serving trace (Poisson, 20 per second)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs one mock per GPU on a single GPU; runs on more "
@@ -189,6 +259,16 @@ def main():
             log(f"bench: 512^3 f64 {extra['ms_per_mock_512_f64']} ms/mock")
         except Exception as e:
             extra["ms_per_mock_512_f64"] = f"failed: {e}"
+        try:
+            extra["batch_64x256"] = run_batch()
+            log(f"bench: batch {extra['batch_64x256']}")
+        except Exception as e:
+            extra["batch_64x256"] = f"failed: {e}"
+        try:
+            extra["serving"] = run_serving(nreq=args.serve_requests)
+            log(f"bench: serving {extra['serving']}")
+        except Exception as e:
+            extra["serving"] = f"failed: {e}"
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle sample ...")

@@ -226,6 +226,23 @@ class EZmock:
         self._assert_return_value()
         return out
 
+    def powspec(self, rsd_fac=0.0, interlace=True, dk=None, nbin=None):
+        """P0 and P2 of the device catalogue (CIC, optional interlacing,
+        line of sight z; rsd_fac > 0 shifts z to redshift space).  Default
+        bins: width k_f = 2 pi / Lbox up to the Nyquist frequency.
+        Returns (k_mean, P0, P2, nmodes) as float64 arrays."""
+        kf = 2 * np.pi / self._Lbox
+        if dk is None:
+            dk = kf
+        if nbin is None:
+            nbin = min(1024, int(np.pi * self._Ng / self._Lbox / dk))
+        out = [np.zeros(int(nbin), dtype=np.float64) for _ in range(4)]
+        self._err.value = self._L.EZmock_ext_powspec(
+            self._ez, float(rsd_fac), bool(interlace), float(dk), int(nbin),
+            *[a.ctypes.data for a in out], C.byref(self._err))
+        self._assert_return_value()
+        return tuple(out)
+
     def field(self, which):
         N = self._Ng ** 3
         codes = {"psi_x": 0, "psi_y": 1, "psi_z": 2, "rho": 3, "rhot": 4}

@@ -26,6 +26,9 @@ EXT_SYMBOLS = [
     "EZmock_ext_kernel_times", "EZmock_ext_reset_kernel_times", "EZmock_ext_sync",
     "EZmock_ext_uniform", "EZmock_ext_fft_c2r", "EZmock_ext_fft_r2c",
     "EZmock_ext_pdf_walk", "EZmock_ext_info", "EZmock_ext_pk_table",
+    "EZmock_ext_powspec", "EZmock_ext_set_seed", "EZmock_ext_footprint",
+    "EZmock_ext_pool_init", "EZmock_ext_pool_submit", "EZmock_ext_pool_wait",
+    "EZmock_ext_result_free", "EZmock_ext_pool_destroy",
 ]
 
 
@@ -98,6 +101,22 @@ def load(precision="f32"):
     L.EZmock_ext_info.argtypes = [vp, vp]
     L.EZmock_ext_pk_table.restype = i32
     L.EZmock_ext_pk_table.argtypes = [vp, vp, vp, vp, i32]
+    L.EZmock_ext_powspec.restype = i32
+    L.EZmock_ext_powspec.argtypes = [vp, dbl, b, dbl, i32, vp, vp, vp, vp, pint]
+    L.EZmock_ext_set_seed.restype = i32
+    L.EZmock_ext_set_seed.argtypes = [vp, u64, pint]
+    L.EZmock_ext_footprint.restype = sz
+    L.EZmock_ext_footprint.argtypes = [i32, i32]
+    L.EZmock_ext_pool_init.restype = vp
+    L.EZmock_ext_pool_init.argtypes = [i32, sz, vp, i32, vp, dbl, dbl, dbl, dbl, dbl, pint]
+    L.EZmock_ext_pool_submit.restype = C.c_long
+    L.EZmock_ext_pool_submit.argtypes = [vp, vp, pint]
+    L.EZmock_ext_pool_wait.restype = i32
+    L.EZmock_ext_pool_wait.argtypes = [vp, C.c_long, vp]
+    L.EZmock_ext_result_free.restype = None
+    L.EZmock_ext_result_free.argtypes = [vp]
+    L.EZmock_ext_pool_destroy.restype = None
+    L.EZmock_ext_pool_destroy.argtypes = [vp]
     L._real = rt
     _CACHE[precision] = L
     return L

@@ -11,6 +11,7 @@
 #include <string.h>
 #include <stdarg.h>
 #include <unistd.h>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -72,6 +73,10 @@ struct Mesh {
   rkey *d_vb = nullptr;
   double *d_rsd = nullptr;
   size_t rsd_cap = 0;
+  // power-spectrum estimator: two assignment grids, a Nyquist plane, sums
+  real *pk_grid[2] = {nullptr, nullptr};
+  cplx *pk_ny = nullptr;
+  double *pk_acc = nullptr;
   real *cat[6] = {nullptr};
   size_t cat_cap = 0, ntracer = 0;
   // last PDF
@@ -84,7 +89,7 @@ struct Mesh {
   std::map<std::string, std::pair<double, long>> times;
 };
 
-long g_pk_version = 0;
+std::atomic<long> g_pk_version{0};   // P(k) tables of all instances
 
 Conf *CONF(EZMOCK *ez) { return (Conf *) ez->conf; }
 RngConf *RNG(EZMOCK *ez) { return (RngConf *) ez->rng; }
@@ -159,6 +164,7 @@ void free_mesh_device(Mesh *m) {
   free_tracer_scratch(m);
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
+  dfree(&m->pk_grid[0]); dfree(&m->pk_grid[1]); dfree(&m->pk_ny); dfree(&m->pk_acc);
   for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
   m->cat_cap = 0;
   for (auto &t : m->pending) { (void) hipEventDestroy(t.a); (void) hipEventDestroy(t.b); }
@@ -1050,6 +1056,75 @@ int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd, int *err) 
   return EZMOCK_SUCCESS;
 }
 
+int EZmock_ext_powspec(EZMOCK *ez, const double rsd_fac, const bool interlace,
+    const double dk, const int nbin, double *k, double *P0, double *P2,
+    double *nmodes, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  const Conf *conf = CONF(ez);
+  if (!m->cat[0] || !k || !P0 || !P2 || !nmodes) return (*err = EZMOCK_ERR_ARG_NULL);
+  if (!(dk > 0) || nbin <= 0 || nbin > 1024) return (*err = EZMOCK_ERR_ARG_NBIN);
+  if (rsd_fac < 0 || rsd_fac > conf->Lbox) return (*err = EZMOCK_ERR_ARG_RSD);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng, plane = (size_t) Ng * Ng;
+  const size_t ntr = m->ntracer;
+  if (ntr == 0) return (*err = EZMOCK_ERR_ARG_NULL);
+  const int ng = interlace ? 2 : 1;
+  int e;
+  for (int g = 0; g < ng; g++)
+    if ((e = dalloc(&m->pk_grid[g], N))) return (*err = e);
+  if ((e = dalloc(&m->pk_ny, plane)) || (e = dalloc(&m->pk_acc, 4 * (size_t) 1024)) ||
+      (e = dalloc(&m->ny, 2 * plane)) || (e = ensure_twiddles(m, Ng)))
+    return (*err = e);
+  for (int g = 0; g < ng; g++) {
+    {
+      Scope s(m, "pk_paint");
+      if (launch_zero(m->pk_grid[g], N * sizeof(real), m->st) != hipSuccess ||
+          launch_pk_paint(m->cat[0], m->cat[1], m->cat[2], m->cat[5], ntr, Ng,
+              conf->Lbox, rsd_fac, g ? 0.5 : 0.0, m->pk_grid[g], m->st) != hipSuccess)
+        return (*err = EZMOCK_ERR_UNKNOWN);
+    }
+    if ((e = r2c_field(m, Ng, m->pk_grid[g]))) return (*err = e);
+    // the Nyquist plane of grid 0 must survive the r2c of grid 1
+    if (g == 0 && interlace &&
+        hipMemcpyAsync(m->pk_ny, m->ny, plane * sizeof(cplx), hipMemcpyDeviceToDevice,
+            m->st) != hipSuccess)
+      return (*err = EZMOCK_ERR_UNKNOWN);
+  }
+  const double L = conf->Lbox;
+  const double kf = 2 * EZ_PI / L;
+  // P(k) = V |X(k)|^2 / ntr^2 for the unnormalised transform X of the counts
+  const double pnorm = L * L * L / ((double) ntr * (double) ntr);
+  {
+    Scope s(m, "pk_bin");
+    const cplx *X1 = (const cplx *) m->pk_grid[0];
+    const cplx *N1 = interlace ? m->pk_ny : m->ny;
+    const cplx *X2 = interlace ? (const cplx *) m->pk_grid[1] : nullptr;
+    const cplx *N2 = interlace ? m->ny : nullptr;
+    if (launch_zero(m->pk_acc, 4 * (size_t) nbin * sizeof(double), m->st) != hipSuccess ||
+        launch_pk_bin(X1, N1, X2, N2, Ng, kf, dk, nbin, pnorm, m->pk_acc, m->st) != hipSuccess)
+      return (*err = EZMOCK_ERR_UNKNOWN);
+  }
+  std::vector<double> acc(4 * (size_t) nbin);
+  if (hipMemcpyAsync(acc.data(), m->pk_acc, acc.size() * sizeof(double),
+      hipMemcpyDeviceToHost, m->st) != hipSuccess) return (*err = EZMOCK_ERR_UNKNOWN);
+  if ((e = sync_and_collect(m))) return (*err = e);
+  const double shot = L * L * L / (double) ntr;
+  for (int b = 0; b < nbin; b++) {
+    const double w = acc[4 * b];
+    nmodes[b] = w;
+    if (w > 0) {
+      k[b] = acc[4 * b + 1] / w;
+      P0[b] = acc[4 * b + 2] / w - shot;
+      P2[b] = 5 * acc[4 * b + 3] / w;
+    }
+    else k[b] = P0[b] = P2[b] = 0;
+  }
+  return EZMOCK_SUCCESS;
+}
+
 int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
@@ -1140,6 +1215,15 @@ int EZmock_ext_pk_table(EZMOCK *ez, double *k, double *P, double *P2,
   memcpy(k, pk->k.data(), sizeof(double) * pk->n);
   memcpy(P, pk->P.data(), sizeof(double) * pk->n);
   memcpy(P2, pk->P2.data(), sizeof(double) * pk->n);
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_set_seed(EZMOCK *ez, const uint64_t seed, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  if (seed == 0) return (*err = EZMOCK_ERR_ARG_SEED);
+  RNG(ez)->seed = seed;
   return EZMOCK_SUCCESS;
 }
 

@@ -90,3 +90,11 @@ hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
     const uint32_t *jtab, double *out, hipStream_t st);
 hipError_t launch_rsd(const real *z, const real *vz, size_t n, double fac,
     double Lbox, double *out, hipStream_t st);
+
+// powspec.hip
+hipError_t launch_pk_paint(const real *x, const real *y, const real *z,
+    const real *vz, size_t n, int Ng, double Lbox, double rsd_fac,
+    double shift, real *grid, hipStream_t st);
+hipError_t launch_pk_bin(const cplx *X1, const cplx *N1, const cplx *X2,
+    const cplx *N2, int Ng, double kf, double dk, int nbin, double pnorm,
+    double *acc, hipStream_t st);

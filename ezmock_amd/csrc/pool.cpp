@@ -1,0 +1,253 @@
+// pool.cpp -- continuous batching of mock requests on one GPU.
+//
+// The reference produces one mock per process call (run_mock.c, or the
+// Python loop of example/fit/fit_EZmock.ipynb over seeds and bias
+// parameters).  A pool runs many such requests on one device at once:
+//   * nslot worker threads, each owning engine instances (EZMOCK) with their
+//     own HIP stream, so the kernels and host round trips of several mocks
+//     overlap on the GPU;
+//   * a shared FIFO of requests; an idle worker takes the first queued
+//     request whose device footprint fits the memory budget next to what
+//     the other workers hold, preferring a request of the grid its cached
+//     instance already has (fields and scratch buffers are kept resident
+//     between requests of one grid size);
+//   * every request is computed by the same calls as a stand-alone mock
+//     (EZmock_set_cosmology, EZmock_setup_linear_pk,
+//     EZmock_create_dens_field, EZmock_ext_populate_device and optionally
+//     EZmock_ext_powspec), so its result does not depend on what else runs.
+// Written against the public C API only.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "common.h"            // EZmock.h and the error codes
+#include "EZmock_ext.h"
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job {
+  long id;
+  EZMOCK_REQUEST req;
+  double t_submit;
+};
+
+struct Done {
+  EZMOCK_RESULT res;
+};
+
+struct Slot {
+  EZMOCK *ez = nullptr;
+  int Ng = 0;
+  double Lbox = 0;
+  size_t held = 0;              // bytes accounted to this slot's instance
+};
+
+}  // namespace
+
+struct EZMOCK_POOL {
+  // model shared by every request
+  std::vector<double> k, P;
+  double z = 0, z_pk = 0, Om = 0, Onu = 0, w = -1;
+  size_t budget = 0;
+  // scheduling
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job> queue;
+  std::map<long, Done> done;
+  std::vector<Slot> slots;
+  size_t held = 0;              // sum of the slots' held bytes
+  long next_id = 1;
+  bool stop = false;
+  std::vector<std::thread> threads;
+};
+
+size_t EZmock_ext_footprint(const int Ngrid, const int with_powspec) {
+  const size_t N = (size_t) Ngrid * Ngrid * Ngrid;
+  const size_t s = sizeof(real);
+  // psi x3, rho, vel4 (x4), rhot; nt (int16); CIC lists (~1.3 N u32, with
+  // head room); candidate keys and sort scratch (<= N keys worst case is
+  // covered by the head room of the estimate); catalogue
+  size_t b = N * s * 9 + 2 * N + 6 * N + N * sizeof(real) + (64u << 20);
+  if (with_powspec) b += 2 * N * s;
+  return b;
+}
+
+namespace {
+
+int run_one(EZMOCK_POOL *p, Slot &sl, const Job &job, EZMOCK_RESULT &r) {
+  const EZMOCK_REQUEST &q = job.req;
+  int err = 0;
+  if (!sl.ez || sl.Ng != q.Ngrid || sl.Lbox != q.Lbox) {
+    if (sl.ez) EZmock_destroy(sl.ez);
+    sl.ez = EZmock_init(q.Lbox, q.Ngrid, 0, q.seed, 0, &err);
+    if (!sl.ez) return err ? err : EZMOCK_ERR_UNKNOWN;
+    sl.Ng = q.Ngrid;
+    sl.Lbox = q.Lbox;
+    EZmock_set_cosmology(sl.ez, 0, 0, true, p->z, p->z_pk, p->Om, p->Onu, p->w, &err);
+    EZmock_setup_linear_pk(sl.ez, p->k.data(), (int) p->k.size(), p->P.data(),
+        nullptr, 0, false, &err);
+    if (err) return err;
+  }
+  EZmock_ext_set_seed(sl.ez, q.seed, &err);
+  EZmock_create_dens_field(sl.ez, nullptr, false, nullptr, false, false, &err);
+  size_t ntr = 0;
+  EZmock_ext_populate_device(sl.ez, q.params, q.nexp, false, &ntr, &err);
+  if (err) return err;
+  r.ntracer = ntr;
+  if (q.pk_nbin > 0) {
+    const int nb = q.pk_nbin;
+    r.k = (double *) malloc(sizeof(double) * nb);
+    r.P0 = (double *) malloc(sizeof(double) * nb);
+    r.P2 = (double *) malloc(sizeof(double) * nb);
+    r.nmodes = (double *) malloc(sizeof(double) * nb);
+    if (!r.k || !r.P0 || !r.P2 || !r.nmodes) return EZMOCK_ERR_MEMORY;
+    EZmock_ext_powspec(sl.ez, q.rsd_fac, true, q.pk_dk, nb, r.k, r.P0, r.P2,
+        r.nmodes, &err);
+    if (err) return err;
+  }
+  if (q.want_catalog && ntr) {
+    for (int i = 0; i < 6; i++) {
+      r.cat[i] = (real *) malloc(sizeof(real) * ntr);
+      if (!r.cat[i]) return EZMOCK_ERR_MEMORY;
+    }
+    EZmock_ext_copy_tracers(sl.ez, r.cat[0], r.cat[1], r.cat[2], r.cat[3],
+        r.cat[4], r.cat[5], ntr, &err);
+  }
+  else EZmock_ext_sync(sl.ez, &err);
+  return err;
+}
+
+// Index of the queued job this slot may take, or -1.
+long pick(EZMOCK_POOL *p, const Slot &sl) {
+  long first_fit = -1;
+  for (size_t i = 0; i < p->queue.size(); i++) {
+    const EZMOCK_REQUEST &q = p->queue[i].req;
+    const size_t need = EZmock_ext_footprint(q.Ngrid, q.pk_nbin > 0);
+    const bool same = sl.ez && sl.Ng == q.Ngrid && sl.Lbox == q.Lbox;
+    const size_t after = p->held - sl.held + std::max(need, same ? sl.held : need);
+    if (after > p->budget && p->held > sl.held) continue;   // wait for memory
+    if (same) return (long) i;
+    if (first_fit < 0) first_fit = (long) i;
+  }
+  return first_fit;
+}
+
+void worker(EZMOCK_POOL *p, int si) {
+  std::unique_lock<std::mutex> lk(p->mu);
+  Slot &sl = p->slots[si];
+  for (;;) {
+    long idx = -1;
+    p->cv_work.wait(lk, [&] {
+      if (p->stop) return true;
+      idx = pick(p, sl);
+      return idx >= 0;
+    });
+    if (p->stop) break;
+    Job job = p->queue[idx];
+    p->queue.erase(p->queue.begin() + idx);
+    const size_t need = EZmock_ext_footprint(job.req.Ngrid, job.req.pk_nbin > 0);
+    const bool same = sl.ez && sl.Ng == job.req.Ngrid && sl.Lbox == job.req.Lbox;
+    const size_t hold = same ? std::max(need, sl.held) : need;
+    p->held = p->held - sl.held + hold;
+    sl.held = hold;
+    lk.unlock();
+    EZMOCK_RESULT r;
+    memset(&r, 0, sizeof(r));
+    r.id = job.id;
+    r.t_submit = job.t_submit;
+    r.t_start = now_s();
+    r.err = run_one(p, sl, job, r);
+    r.t_end = now_s();
+    lk.lock();
+    if (r.err && sl.ez) {            // an instance with a sticky error is dropped
+      EZmock_destroy(sl.ez);
+      sl.ez = nullptr;
+      p->held -= sl.held;
+      sl.held = 0;
+    }
+    p->done[job.id] = Done{r};
+    p->cv_done.notify_all();
+    p->cv_work.notify_all();         // memory accounting may admit others
+  }
+}
+
+}  // namespace
+
+EZMOCK_POOL *EZmock_ext_pool_init(const int nslot, const size_t mem_budget,
+    const double *k, const int nk, const double *Pk, const double z,
+    const double z_pk, const double Omega_m, const double Omega_nu,
+    const double w, int *err) {
+  if (!err) return nullptr;
+  if (*err) return nullptr;
+  if (nslot < 1 || nslot > 64) { *err = EZMOCK_ERR_ARG_THREAD; return nullptr; }
+  if (!k || !Pk || nk < 4) { *err = EZMOCK_ERR_ARG_PK; return nullptr; }
+  EZMOCK_POOL *p = new EZMOCK_POOL();
+  p->k.assign(k, k + nk);
+  p->P.assign(Pk, Pk + nk);
+  p->z = z; p->z_pk = z_pk; p->Om = Omega_m; p->Onu = Omega_nu; p->w = w;
+  p->budget = mem_budget;
+  p->slots.resize(nslot);
+  for (int i = 0; i < nslot; i++) p->threads.emplace_back(worker, p, i);
+  return p;
+}
+
+long EZmock_ext_pool_submit(EZMOCK_POOL *p, const EZMOCK_REQUEST *req, int *err) {
+  if (!err) return -1;
+  if (*err) return -1;
+  if (!p || !req) { *err = EZMOCK_ERR_ARG_NULL; return -1; }
+  if (req->seed == 0) { *err = EZMOCK_ERR_ARG_SEED; return -1; }
+  if (EZmock_ext_footprint(req->Ngrid, req->pk_nbin > 0) > p->budget) {
+    *err = EZMOCK_ERR_MEMORY;
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(p->mu);
+  Job j;
+  j.id = p->next_id++;
+  j.req = *req;
+  j.t_submit = now_s();
+  p->queue.push_back(j);
+  p->cv_work.notify_all();
+  return j.id;
+}
+
+int EZmock_ext_pool_wait(EZMOCK_POOL *p, const long id, EZMOCK_RESULT *res) {
+  if (!p || !res) return EZMOCK_ERR_ARG_NULL;
+  std::unique_lock<std::mutex> lk(p->mu);
+  p->cv_done.wait(lk, [&] { return p->done.count(id) > 0; });
+  *res = p->done[id].res;
+  p->done.erase(id);
+  return res->err;
+}
+
+void EZmock_ext_result_free(EZMOCK_RESULT *res) {
+  if (!res) return;
+  free(res->k); free(res->P0); free(res->P2); free(res->nmodes);
+  for (int i = 0; i < 6; i++) free(res->cat[i]);
+  memset(res, 0, sizeof(*res));
+}
+
+void EZmock_ext_pool_destroy(EZMOCK_POOL *p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->stop = true;
+    p->cv_work.notify_all();
+  }
+  for (auto &t : p->threads) t.join();
+  for (auto &sl : p->slots)
+    if (sl.ez) EZmock_destroy(sl.ez);
+  for (auto &d : p->done) EZmock_ext_result_free(&d.second.res);
+  delete p;
+}

@@ -73,6 +73,74 @@ int EZmock_ext_info(EZMOCK *ez, double *out8);
 int EZmock_ext_pk_table(EZMOCK *ez, double *k, double *P, double *P2,
     const int n);
 
+/* Power-spectrum multipoles of the device-resident catalogue of the last
+   populate call, for calibration loops (the reference fits its mocks with an
+   external interlaced-CIC estimator, example/fit/conf/powspec.conf).  Tracers
+   are CIC-assigned to the mock's Ng^3 grid (a second grid shifted by half a
+   cell when `interlace`); z is shifted to redshift space by rsd_fac*vz and
+   wrapped once into [0, Lbox) (rsd_fac = 0: real space, line of sight z).
+   The window of the assignment is divided out.  Bin b holds the modes with
+   b*dk <= |k| < (b+1)*dk (nbin <= 1024): mean k, P0 (shot noise V/ntracer
+   subtracted), P2 and the number of modes (the +k/-k pair counts twice). */
+int EZmock_ext_powspec(EZMOCK *ez, const double rsd_fac, const bool interlace,
+    const double dk, const int nbin, double *k, double *P0, double *P2,
+    double *nmodes, int *err);
+
+/* Re-seed the instance's random streams (the next create_dens_field and
+   populate use `seed`; 0 is rejected as by EZmock_init). */
+int EZmock_ext_set_seed(EZMOCK *ez, const uint64_t seed, int *err);
+
+/* ---- request pool: continuous batching of mocks on one GPU ----
+   nslot worker threads, each with its own engine instance and HIP stream,
+   take queued requests as slots and device memory (mem_budget bytes, by the
+   estimate of EZmock_ext_footprint) free up; requests of one grid size
+   reuse the resident buffers of a worker.  Every request runs the same
+   calls as a stand-alone mock with the pool's cosmology (set_cosmology with
+   eval = true) and linear P(k) (no BAO modification, linear spline). */
+typedef struct EZMOCK_POOL EZMOCK_POOL;
+
+typedef struct {
+  int Ngrid;                    /* grid size                               */
+  double Lbox;                  /* box size                                */
+  uint64_t seed;                /* MRG32k3a seed (> 0)                     */
+  real params[4];               /* rho_c, rho_exp, pdf_base, sigma_v       */
+  size_t nexp;                  /* expected number of tracers              */
+  double rsd_fac;               /* P(k): redshift-space shift (0 = real)   */
+  double pk_dk;                 /* P(k): bin width                         */
+  int pk_nbin;                  /* P(k): number of bins (0 = no P(k))      */
+  int want_catalog;             /* copy the catalogue to the result        */
+} EZMOCK_REQUEST;
+
+typedef struct {
+  long id;
+  int err;                      /* 0 or the engine's error code            */
+  size_t ntracer;
+  double t_submit, t_start, t_end;   /* steady-clock seconds               */
+  double *k, *P0, *P2, *nmodes; /* pk_nbin each (malloc'd) or NULL        */
+  real *cat[6];                 /* x, y, z, vx, vy, vz (malloc'd) or NULL */
+} EZMOCK_RESULT;
+
+/* Device bytes accounted to one request of grid Ngrid. */
+size_t EZmock_ext_footprint(const int Ngrid, const int with_powspec);
+
+EZMOCK_POOL *EZmock_ext_pool_init(const int nslot, const size_t mem_budget,
+    const double *k, const int nk, const double *Pk, const double z,
+    const double z_pk, const double Omega_m, const double Omega_nu,
+    const double w, int *err);
+
+/* Queue a request; returns its id (> 0), or -1 with *err set. */
+long EZmock_ext_pool_submit(EZMOCK_POOL *pool, const EZMOCK_REQUEST *req,
+    int *err);
+
+/* Block until request `id` is done; fills *res (free its arrays with
+   EZmock_ext_result_free) and returns res->err. */
+int EZmock_ext_pool_wait(EZMOCK_POOL *pool, const long id, EZMOCK_RESULT *res);
+
+void EZmock_ext_result_free(EZMOCK_RESULT *res);
+
+/* Finishes the running requests, drops the queued ones, frees everything. */
+void EZmock_ext_pool_destroy(EZMOCK_POOL *pool);
+
 /* Wait for all device work of the instance. */
 int EZmock_ext_sync(EZMOCK *ez, int *err);
 
