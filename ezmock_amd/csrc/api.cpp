@@ -277,15 +277,10 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   return 0;
 }
 
-// Inverse 3-D c2r of component `comp` of the generated spectrum (rhok in
-// m->rho viewed as complex, Nyquist plane in m->ny) into out.
-int c2r_component(Mesh *m, int Ng, int comp, real *out) {
+// The (j, rows) passes of the c2r of a displacement component whose first
+// pass (axis i, fft_first_pass3 from the generated spectrum) is done.
+int c2r_finish(Mesh *m, int Ng, real *out) {
   cplx *buf = (cplx *) out;
-  {
-    Scope s(m, "fft_pass_i");
-    KCHK(fft_strided_pass(buf, (const cplx *) m->rho, m->ny, m->twN, Ng, 0,
-        +1, 1, comp, m->st));
-  }
   {
     Scope s(m, "fft_pass_j");
     KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, m->st));
@@ -848,8 +843,16 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     // the generated Nyquist plane lives in ny[plane, 2 plane)
     cplx *ny_save = m->ny;
     m->ny = m->ny + plane;
+    {
+      Scope s(m, "fft_pass_i");
+      if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
+          rhok, m->ny, m->twN, Ng, m->st) != hipSuccess) {
+        m->ny = ny_save;
+        return (*err = EZMOCK_ERR_UNKNOWN);
+      }
+    }
     for (int c = 0; c < 3; c++) {
-      if ((e = c2r_component(m, Ng, c, m->psi[c]))) { m->ny = ny_save; return (*err = e); }
+      if ((e = c2r_finish(m, Ng, m->psi[c]))) { m->ny = ny_save; return (*err = e); }
     }
     m->ny = ny_save;
     if (delta) {

@@ -379,7 +379,8 @@ __global__ void k_cursor_init(uint64_t *__restrict__ off,
   else if (i == nk) off[nk] = *total;
 }
 
-// Tile shape: 8 x 8 x 32 cells when the grid allows it.
+// Tile shape: 8 x 8 x 16 cells when the grid allows it (32 KB of double
+// accumulators for density + momentum: five workgroups per CU).
 static int tile_dim(int Ng, int pref) {
   int d = pref;
   while (d > 1 && Ng % d) d >>= 1;
@@ -392,7 +393,7 @@ CicGeom cic_geometry(int Ng, real igs) {
   g.igs = igs;
   g.TX = tile_dim(Ng, 8);
   g.TY = tile_dim(Ng, 8);
-  g.TZ = tile_dim(Ng, 32);
+  g.TZ = tile_dim(Ng, 16);
   g.lx = __builtin_ctz(g.TX); g.ly = __builtin_ctz(g.TY); g.lz = __builtin_ctz(g.TZ);
   g.ntx = Ng / g.TX; g.nty = Ng / g.TY; g.ntz = Ng / g.TZ;
   g.N = (size_t) Ng * Ng * Ng;
@@ -451,7 +452,7 @@ hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
 }
 
 // Particle indices into the lists of the tiles they touch (list: off[nkeys]
-// u32, between N and 8 N; about 1.3 N for 8 x 8 x 32 tiles).
+// u32, between N and 8 N; about 1.35 N for 8 x 8 x 16 tiles).
 hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, uint64_t *cursor, uint32_t *list, hipStream_t st) {
   const unsigned nb = (unsigned) ((g.N + CIC_PPB - 1) / CIC_PPB);

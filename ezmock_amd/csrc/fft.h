@@ -17,3 +17,8 @@ hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
 // row), -1 = r2c.  twN: exp(+2 pi i m / Ng), twh: exp(+2 pi i m / (Ng/2)).
 hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
     int sign, hipStream_t st);
+
+// First c2r pass (axis i) of the three displacement components rhok * n_a,
+// a = x, y, z, into out0..out2 (spectrum read once).
+hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, hipStream_t st);

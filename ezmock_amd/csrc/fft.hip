@@ -217,25 +217,44 @@ __device__ __forceinline__ C spec_load(const C *__restrict__ rhok,
   return C{h0.x - hn.y, h0.y + hn.x};
 }
 
-// MODE 0: in place on buf; MODE 1: first pass from the generated spectrum.
-// Grid: x = kk tile, y = outer line.  The transform index n walks `stride`.
+// MODE 0: in place on buf; MODE 1: first pass from the generated spectrum
+// (either axis).  Grid: x = kk tile, y = outer line.  The transform index n
+// walks `stride`.
 template <class C, int L, int TK, int NT, int SIGN, int MODE>
 __global__ void __launch_bounds__(NT)
 k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
     const C *__restrict__ rhok_ny, const C *__restrict__ tw, int Ng, int Ngh,
-    size_t stride, size_t outer_stride, int comp) {
+    size_t stride, size_t outer_stride, int comp, int axis) {
   __shared__ C lds[L * TK];
   const int kk0 = blockIdx.x * TK;
   const int o = blockIdx.y;
   const size_t base = (size_t) o * outer_stride + kk0;
   constexpr int E = L * TK;
+  if (MODE == 0) {
 #pragma unroll 4
-  for (int e = threadIdx.x; e < E; e += NT) {
-    const int n = e / TK, c = e % TK;
-    C v;
-    if (MODE == 0) v = buf[base + (size_t) n * stride + c];
-    else v = spec_load(rhok, rhok_ny, n, o, kk0 + c, Ng, Ngh, comp);
-    lds[n * TK + c] = v;
+    for (int e = threadIdx.x; e < E; e += NT) {
+      const int n = e / TK, c = e % TK;
+      lds[n * TK + c] = buf[base + (size_t) n * stride + c];
+    }
+  }
+  else {
+    // rhok has buf's layout: plain, batched loads times n_a; the kz = 0 /
+    // Nyquist slot (kk = 0, first column tile only) is redone afterwards
+    const int ko = (o <= Ngh) ? o : o - Ng;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < E; e += NT) {
+      const int n = e / TK, c = e % TK;
+      const int kn = (n <= Ngh) ? n : n - Ng;
+      const C h = rhok[base + (size_t) n * stride + c];
+      lds[n * TK + c] = axis ? spec_mul(h, comp, ko, kn, kk0 + c)
+                             : spec_mul(h, comp, kn, ko, kk0 + c);
+    }
+    if (kk0 == 0) {                     // uniform over the block
+      __syncthreads();
+      for (int n = threadIdx.x; n < L; n += NT)
+        lds[n * TK] = axis ? spec_load(rhok, rhok_ny, o, n, 0, Ng, Ngh, comp)
+                           : spec_load(rhok, rhok_ny, n, o, 0, Ng, Ngh, comp);
+    }
   }
   __syncthreads();
   lds_fft<C, L, TK, NT, SIGN, TK, 1, true>(lds, tw);
@@ -243,6 +262,65 @@ k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
   for (int e = threadIdx.x; e < E; e += NT) {
     const int n = e / TK, c = e % TK;
     buf[base + (size_t) n * stride + c] = lds[n * TK + c];
+  }
+}
+
+// First c2r pass (axis i) of the three displacement components at once:
+// the spectrum tile is loaded once into registers, then for a = x, y, z it
+// is multiplied by n_a into LDS (kz = 0 / Nyquist slot as spec_load), the
+// i transform runs in LDS and the tile is stored to out[a].  Reads the
+// spectrum once instead of three times.
+template <class C, int L, int TK, int NT>
+__global__ void __launch_bounds__(NT)
+k_fft_first3(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
+    const C *__restrict__ rhok, const C *__restrict__ rhok_ny,
+    const C *__restrict__ tw, int Ng, int Ngh) {
+  __shared__ C lds[L * TK];
+  constexpr int E = L * TK;
+  constexpr int IT = (E + NT - 1) / NT;
+  const int kk0 = blockIdx.x * TK;
+  const int o = blockIdx.y;                 // j
+  const size_t stride = (size_t) Ng * Ngh;
+  const size_t base = (size_t) o * Ngh + kk0;
+  const int ko = (o <= Ngh) ? o : o - Ng;
+  C h[IT];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * NT;
+    if (IT * NT == E || e < E) {
+      const int n = e / TK, c = e % TK;
+      h[it] = rhok[base + (size_t) n * stride + c];
+    }
+  }
+  C *outs[3] = {out0, out1, out2};
+#pragma unroll 1
+  for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      if (IT * NT == E || e < E) {
+        const int n = e / TK, c = e % TK;
+        const int kn = (n <= Ngh) ? n : n - Ng;
+        lds[n * TK + c] = spec_mul(h[it], comp, kn, ko, kk0 + c);
+      }
+    }
+    if (kk0 == 0) {                         // uniform over the block
+      __syncthreads();
+      for (int n = threadIdx.x; n < L; n += NT)
+        lds[n * TK] = spec_load(rhok, rhok_ny, n, o, 0, Ng, Ngh, comp);
+    }
+    __syncthreads();
+    lds_fft<C, L, TK, NT, +1, TK, 1, true>(lds, tw);
+    C *out = outs[comp];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      if (IT * NT == E || e < E) {
+        const int n = e / TK, c = e % TK;
+        out[base + (size_t) n * stride + c] = lds[n * TK + c];
+      }
+    }
+    __syncthreads();                        // before the next component's fill
   }
 }
 
@@ -399,10 +477,10 @@ static hipError_t launch_strided_tk(cplx *buf, const cplx *rhok,
   dim3 grid(Ngh / TK, Ng);
   if (mode == 0)
     hipLaunchKernelGGL((k_fft_strided<cplx, L, TK, NT, SIGN, 0>), grid, dim3(NT), 0, st,
-        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp);
+        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp, axis);
   else
     hipLaunchKernelGGL((k_fft_strided<cplx, L, TK, NT, SIGN, 1>), grid, dim3(NT), 0, st,
-        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp);
+        buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp, axis);
   return hipGetLastError();
 }
 
@@ -416,6 +494,27 @@ static hipError_t launch_strided(cplx *buf, const cplx *rhok,
   if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_strided_tk<L, 4, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
   if (Ngh % 2 == 0) return launch_strided_tk<L, 2, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
   return launch_strided_tk<L, 1, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+}
+
+template <int L, int T>
+static hipError_t launch_first3_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
+  constexpr int E = L * T;
+  constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
+  hipLaunchKernelGGL((k_fft_first3<cplx, L, T, NT>), dim3(Ngh / T, Ng), dim3(NT), 0, st,
+      o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh);
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
+  constexpr int T = StridedCfg<L>::TKMAX;
+  if (Ngh % T == 0) return launch_first3_tk<L, T>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
+  if constexpr (T >= 8) if (Ngh % (T / 2) == 0) return launch_first3_tk<L, T / 2>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
+  if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_first3_tk<L, 4>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
+  if (Ngh % 2 == 0) return launch_first3_tk<L, 2>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
+  return launch_first3_tk<L, 1>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
 }
 
 template <int Lh>
@@ -463,6 +562,17 @@ hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
   const size_t nrows = (size_t) Ng * Ng;
   switch (Ng / 2) {
 #define X(n) case n / 2: return launch_rows<n / 2>(buf, twN, twh, nrows, sign, st);
+    EZ_FFT_SIZES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, hipStream_t st) {
+  const int Ngh = Ng / 2;
+  switch (Ng) {
+#define X(n) case n: return launch_first3<n>(out0, out1, out2, rhok, rhok_ny, tw, Ng, Ngh, st);
     EZ_FFT_SIZES(X)
 #undef X
     default: return hipErrorInvalidValue;
