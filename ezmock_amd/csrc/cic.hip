@@ -27,6 +27,12 @@
 #define CIC_TPB 256
 #define CIC_PPT 8
 #define CIC_PPB (CIC_TPB * CIC_PPT)
+#ifndef CIC_TILE_TPB
+#define CIC_TILE_TPB 256
+#endif
+#ifndef CIC_TILES_PER_WG
+#define CIC_TILES_PER_WG 1
+#endif
 
 struct ParticleCell {
   real x, y, z;                 // displaced grid coordinates
@@ -273,31 +279,36 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
 // cell {rho_c, u0, u1, u2} with u_a = m_a / rho_c where rho_c > REAL_TOL
 // (the same IEEE division the gather would do per corner).
 template <int MOM, int TXX, int TYY, int TZZ>
-__global__ void __launch_bounds__(CIC_TPB)
+__global__ void __launch_bounds__(CIC_TILE_TPB)
 k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
     const real *__restrict__ p2, CicGeom g, const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ list, real *__restrict__ rho,
-    const real *__restrict__ rho_norm, real4 *__restrict__ vel) {
+    const real *__restrict__ rho_norm, real4 *__restrict__ vel, int tpw) {
   constexpr int TC = TXX * TYY * TZZ;
   constexpr int NF = MOM ? 4 : 1;
   // double accumulators: LDS double atomics are several times faster than
   // float ones on gfx950, and the float result is the correctly rounded
   // sum of the float contributions
   __shared__ double acc[NF][TC];
-  for (int c = threadIdx.x; c < NF * TC; c += CIC_TPB) (&acc[0][0])[c] = 0;
+  // tpw consecutive tiles (z-neighbours, whose particle rows share cache
+  // lines) per workgroup, the workgroups of one XCD on consecutive groups
+  const int ntile = g.ntx * g.nty * g.ntz;
+  const int t0 = (int) xcd_item(blockIdx.x, gridDim.x) * tpw;
+  const int t1 = (t0 + tpw < ntile) ? t0 + tpw : ntile;
+  for (int t = t0; t < t1; t++) {
+  for (int c = threadIdx.x; c < NF * TC; c += CIC_TILE_TPB) (&acc[0][0])[c] = 0;
   __syncthreads();
-  const int t = blockIdx.x;
   const int tz = t % g.ntz, ty = (t / g.ntz) % g.nty, tx = t / (g.ntz * g.nty);
   const int Ng = g.Ng;
   const int ox = tx * TXX, oy = ty * TYY, oz = tz * TZZ;
   const uint64_t s0 = off[t], e0 = off[t + 1];
   constexpr int U = 4;
-  for (uint64_t qb = s0; qb < e0; qb += (uint64_t) U * CIC_TPB) {
+  for (uint64_t qb = s0; qb < e0; qb += (uint64_t) U * CIC_TILE_TPB) {
   size_t pv[U];
   real qq[U][3];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const uint64_t q = qb + (uint64_t) u * CIC_TPB + threadIdx.x;
+    const uint64_t q = qb + (uint64_t) u * CIC_TILE_TPB + threadIdx.x;
     pv[u] = (q < e0) ? (size_t) list[q] : (size_t) -1;
   }
 #pragma unroll
@@ -354,7 +365,7 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
   }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < TC; c += CIC_TPB) {
+  for (int c = threadIdx.x; c < TC; c += CIC_TILE_TPB) {
     const int lz = c % TZZ, ly = (c / TZZ) % TYY, lx = c / (TZZ * TYY);
     const size_t gi = ((size_t) (ox + lx) * Ng + (oy + ly)) * Ng + (oz + lz);
     const real r = (real) acc[0][c];
@@ -369,6 +380,8 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
       }
       vel[gi] = real4{rc, u[0], u[1], u[2]};
     }
+  }
+  __syncthreads();                      // before the next tile's zeroing
   }
 }
 
@@ -409,8 +422,9 @@ static void launch_tile(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
     const real *rho_norm, real4 *vel, hipStream_t st) {
   const unsigned nt = (unsigned) (g.ntx * g.nty * g.ntz);
-  hipLaunchKernelGGL((k_cic_tile<MOM, A, B, C>), dim3(nt), dim3(CIC_TPB), 0, st,
-      p0, p1, p2, g, off, list, rho, rho_norm, vel);
+  const int tpw = CIC_TILES_PER_WG;
+  hipLaunchKernelGGL((k_cic_tile<MOM, A, B, C>), dim3((nt + tpw - 1) / tpw),
+      dim3(CIC_TILE_TPB), 0, st, p0, p1, p2, g, off, list, rho, rho_norm, vel, tpw);
 }
 
 template <int MOM>

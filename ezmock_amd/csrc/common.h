@@ -108,3 +108,15 @@ __host__ __device__ __forceinline__ uint32_t fastdiv_qr(uint32_t n, const FastDi
   r = n - q * f.d;
   return q;
 }
+
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
+// MI355X (workgroup b runs on XCD b % 8), each with its own L2.  For kernels
+// whose neighbouring work items share cache lines (gathers), map workgroup b
+// to an item so that every XCD gets one contiguous range of items:
+// XCD x takes items [x*per + min(x, rem), ... + per + (x < rem)).
+#define EZ_NUM_XCD 8
+__device__ __forceinline__ unsigned xcd_item(unsigned b, unsigned n) {
+  const unsigned per = n / EZ_NUM_XCD, rem = n % EZ_NUM_XCD;
+  const unsigned x = b % EZ_NUM_XCD, q = b / EZ_NUM_XCD;
+  return x * per + (x < rem ? x : rem) + q;
+}

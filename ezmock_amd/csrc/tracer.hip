@@ -639,7 +639,7 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     real *__restrict__ vy, real *__restrict__ vz) {
   __shared__ MrgState base;
   const size_t npair = (ntot + 1) / 2;
-  const size_t q0blk = (size_t) blockIdx.x * TPB * PPT;
+  const size_t q0blk = (size_t) xcd_item(blockIdx.x, gridDim.x) * TPB * PPT;
   if (threadIdx.x == 0) {
     MrgState g = seed;
     mrg_jump(g, base_draw + 6 * q0blk, jtab);
