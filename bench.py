@@ -41,7 +41,7 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
     F = s * N
     table = {
         "gen_spectrum": F,                     # write the spectrum once
-        "fft_pass_i": 2 * F,                   # read + write the field
+        "fft_pass_i": 4 * F,                   # read the spectrum, write 3 components
         "fft_pass_j": 2 * F,
         "fft_rows_c2r": 2 * F,
         "bias_count": F,

@@ -49,6 +49,7 @@ struct Mesh {
   real *psi[3] = {nullptr, nullptr, nullptr};
   real *rho = nullptr;          // also the spectrum buffer before the CIC
   cplx *ny = nullptr;           // kz = Nyquist plane (+ r2c scratch), 2 Ng^2
+  cplx *slot0 = nullptr;        // packed kz = 0 / Nyquist slots, 3 Ng^2
   real *rhot = nullptr;
   int16_t *nt = nullptr;
   real4 *vel4 = nullptr;        // {rho, u0, u1, u2} per cell, for the velocities
@@ -155,7 +156,7 @@ void free_tracer_scratch(Mesh *m) {
 void free_mesh_device(Mesh *m) {
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
   dfree(&m->vel4);
-  dfree(&m->rho); dfree(&m->ny); dfree(&m->rhot); dfree(&m->nt);
+  dfree(&m->rho); dfree(&m->ny); dfree(&m->slot0); dfree(&m->rhot); dfree(&m->nt);
   dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
   dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
   dfree(&m->hist); dfree(&m->counter);
@@ -843,10 +844,18 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     // the generated Nyquist plane lives in ny[plane, 2 plane)
     cplx *ny_save = m->ny;
     m->ny = m->ny + plane;
+    if ((e = dalloc(&m->slot0, 3 * plane))) { m->ny = ny_save; return (*err = e); }
+    {
+      Scope s(m, "fft_slot0");
+      if (fft_slot0_pack(rhok, m->ny, m->slot0, Ng, m->st) != hipSuccess) {
+        m->ny = ny_save;
+        return (*err = EZMOCK_ERR_UNKNOWN);
+      }
+    }
     {
       Scope s(m, "fft_pass_i");
       if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
-          rhok, m->ny, m->twN, Ng, m->st) != hipSuccess) {
+          rhok, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
         m->ny = ny_save;
         return (*err = EZMOCK_ERR_UNKNOWN);
       }

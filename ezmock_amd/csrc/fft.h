@@ -18,7 +18,12 @@ hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
 hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
     int sign, hipStream_t st);
 
+// Packed kz = 0 / Nyquist slots (kk = 0) of the three displacement
+// components, slot0[comp][j][i] (3 Ng^2 complex), for fft_first_pass3.
+hipError_t fft_slot0_pack(const cplx *rhok, const cplx *rhok_ny, cplx *slot0,
+    int Ng, hipStream_t st);
+
 // First c2r pass (axis i) of the three displacement components rhok * n_a,
-// a = x, y, z, into out0..out2 (spectrum read once).
+// a = x, y, z, into out0..out2 (spectrum read once; kk = 0 from slot0).
 hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
-    const cplx *rhok_ny, const cplx *tw, int Ng, hipStream_t st);
+    const cplx *slot0, const cplx *tw, int Ng, hipStream_t st);

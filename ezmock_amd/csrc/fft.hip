@@ -193,6 +193,21 @@ __device__ __forceinline__ C spec_mul(C h, int comp, int ki, int kj, int kz) {
   return C{h.x * n, h.y * n};
 }
 
+// Packed kk = 0 slot from the kz = 0 values at k and -k (p0, q0) and the
+// Nyquist values (p1, q1), for component comp.
+template <class C>
+__device__ __forceinline__ C spec_pack(C p0, C q0, C p1, C q1, int comp, int ki,
+    int kj, int kni, int knj, int Ngh) {
+  using R = decltype(C{}.x);
+  const C a0 = spec_mul(p0, comp, ki, kj, 0);
+  const C b0 = spec_mul(q0, comp, kni, knj, 0);
+  const C a1 = spec_mul(p1, comp, ki, kj, Ngh);
+  const C b1 = spec_mul(q1, comp, kni, knj, Ngh);
+  const C h0 = C{(a0.x + b0.x) * (R) 0.5, (a0.y - b0.y) * (R) 0.5};
+  const C hn = C{(a1.x + b1.x) * (R) 0.5, (a1.y - b1.y) * (R) 0.5};
+  return C{h0.x - hn.y, h0.y + hn.x};
+}
+
 // kz = 0 and kz = Ny planes: Hermitian part (H(k) + conj H(-k)) / 2 of each
 // plane (exact for the generator's mirrored modes; anti-Hermitian lines of
 // psi_x, psi_y and the psi_z Nyquist plane vanish), packed as H0 + i HNy.
@@ -208,13 +223,8 @@ __device__ __forceinline__ C spec_load(const C *__restrict__ rhok,
   const int ni = (i == 0) ? 0 : Ng - i, nj = (j == 0) ? 0 : Ng - j;
   const int kni = (ni <= Ngh) ? ni : ni - Ng, knj = (nj <= Ngh) ? nj : nj - Ng;
   const size_t p = (size_t) i * Ng + j, q = (size_t) ni * Ng + nj;
-  const C a0 = spec_mul(rhok[p * Ngh], comp, ki, kj, 0);
-  const C b0 = spec_mul(rhok[q * Ngh], comp, kni, knj, 0);
-  const C a1 = spec_mul(rhok_ny[p], comp, ki, kj, Ngh);
-  const C b1 = spec_mul(rhok_ny[q], comp, kni, knj, Ngh);
-  const C h0 = C{(a0.x + b0.x) * (R) 0.5, (a0.y - b0.y) * (R) 0.5};
-  const C hn = C{(a1.x + b1.x) * (R) 0.5, (a1.y - b1.y) * (R) 0.5};
-  return C{h0.x - hn.y, h0.y + hn.x};
+  return spec_pack(rhok[p * Ngh], rhok[q * Ngh], rhok_ny[p], rhok_ny[q], comp,
+      ki, kj, kni, knj, Ngh);
 }
 
 // MODE 0: in place on buf; MODE 1: first pass from the generated spectrum
@@ -265,15 +275,53 @@ k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
   }
 }
 
+// kz = 0 / Nyquist slot (kk = 0) of the three components, spec_load for
+// every (i, j), stored as slot0[comp][j][i] so that the first pass reads
+// its column contiguously: the four mirrored loads of a slot are strided
+// across the whole spectrum and are done once here instead of in the
+// strided pass.
+#define SP_T 16
+__global__ void __launch_bounds__(SP_T * SP_T)
+k_slot0_pack(const cplx *__restrict__ rhok, const cplx *__restrict__ rhok_ny,
+    int Ng, cplx *__restrict__ slot0) {
+  __shared__ cplx tile[3][SP_T][SP_T + 1];
+  const int Ngh = Ng >> 1;
+  const size_t plane = (size_t) Ng * Ng;
+  const int i0 = blockIdx.x * SP_T, j0 = blockIdx.y * SP_T;
+  {
+    // loads with j fastest: 2 KB apart, within one page
+    const int i = i0 + (int) threadIdx.x / SP_T, j = j0 + (int) threadIdx.x % SP_T;
+    if (i < Ng && j < Ng) {
+      const int ni = (i == 0) ? 0 : Ng - i, nj = (j == 0) ? 0 : Ng - j;
+      const int ki = (i <= Ngh) ? i : i - Ng, kj = (j <= Ngh) ? j : j - Ng;
+      const int kni = (ni <= Ngh) ? ni : ni - Ng, knj = (nj <= Ngh) ? nj : nj - Ng;
+      const size_t p = (size_t) i * Ng + j, q = (size_t) ni * Ng + nj;
+      const cplx p0 = rhok[p * Ngh], q0 = rhok[q * Ngh];
+      const cplx p1 = rhok_ny[p], q1 = rhok_ny[q];
+#pragma unroll
+      for (int comp = 0; comp < 3; comp++)
+        tile[comp][i - i0][j - j0] = spec_pack(p0, q0, p1, q1, comp, ki, kj, kni, knj, Ngh);
+    }
+  }
+  __syncthreads();
+  // stores with i fastest: slot0[comp][j][i]
+  const int i = i0 + (int) threadIdx.x % SP_T, j = j0 + (int) threadIdx.x / SP_T;
+  if (i < Ng && j < Ng) {
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++)
+      slot0[comp * plane + (size_t) j * Ng + i] = tile[comp][i - i0][j - j0];
+  }
+}
+
 // First c2r pass (axis i) of the three displacement components at once:
 // the spectrum tile is loaded once into registers, then for a = x, y, z it
-// is multiplied by n_a into LDS (kz = 0 / Nyquist slot as spec_load), the
+// is multiplied by n_a into LDS (the kk = 0 slot from k_slot0_pack), the
 // i transform runs in LDS and the tile is stored to out[a].  Reads the
 // spectrum once instead of three times.
 template <class C, int L, int TK, int NT>
 __global__ void __launch_bounds__(NT)
 k_fft_first3(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
-    const C *__restrict__ rhok, const C *__restrict__ rhok_ny,
+    const C *__restrict__ rhok, const C *__restrict__ slot0,
     const C *__restrict__ tw, int Ng, int Ngh) {
   __shared__ C lds[L * TK];
   constexpr int E = L * TK;
@@ -305,9 +353,9 @@ k_fft_first3(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
       }
     }
     if (kk0 == 0) {                         // uniform over the block
+      const C *col = slot0 + ((size_t) comp * Ng + o) * Ng;
       __syncthreads();
-      for (int n = threadIdx.x; n < L; n += NT)
-        lds[n * TK] = spec_load(rhok, rhok_ny, n, o, 0, Ng, Ngh, comp);
+      for (int n = threadIdx.x; n < L; n += NT) lds[n * TK] = col[n];
     }
     __syncthreads();
     lds_fft<C, L, TK, NT, +1, TK, 1, true>(lds, tw);
@@ -498,23 +546,23 @@ static hipError_t launch_strided(cplx *buf, const cplx *rhok,
 
 template <int L, int T>
 static hipError_t launch_first3_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
-    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
+    const cplx *slot0, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
   constexpr int E = L * T;
   constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
   hipLaunchKernelGGL((k_fft_first3<cplx, L, T, NT>), dim3(Ngh / T, Ng), dim3(NT), 0, st,
-      o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh);
+      o0, o1, o2, rhok, slot0, tw, Ng, Ngh);
   return hipGetLastError();
 }
 
 template <int L>
 static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
-    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
+    const cplx *slot0, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
   constexpr int T = StridedCfg<L>::TKMAX;
-  if (Ngh % T == 0) return launch_first3_tk<L, T>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
-  if constexpr (T >= 8) if (Ngh % (T / 2) == 0) return launch_first3_tk<L, T / 2>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
-  if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_first3_tk<L, 4>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
-  if (Ngh % 2 == 0) return launch_first3_tk<L, 2>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
-  return launch_first3_tk<L, 1>(o0, o1, o2, rhok, rhok_ny, tw, Ng, Ngh, st);
+  if (Ngh % T == 0) return launch_first3_tk<L, T>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+  if constexpr (T >= 8) if (Ngh % (T / 2) == 0) return launch_first3_tk<L, T / 2>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+  if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_first3_tk<L, 4>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+  if (Ngh % 2 == 0) return launch_first3_tk<L, 2>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+  return launch_first3_tk<L, 1>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
 }
 
 template <int Lh>
@@ -568,11 +616,19 @@ hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
   }
 }
 
+hipError_t fft_slot0_pack(const cplx *rhok, const cplx *rhok_ny, cplx *slot0,
+    int Ng, hipStream_t st) {
+  const unsigned nb = (unsigned) ((Ng + SP_T - 1) / SP_T);
+  hipLaunchKernelGGL(k_slot0_pack, dim3(nb, nb), dim3(SP_T * SP_T), 0, st,
+      rhok, rhok_ny, Ng, slot0);
+  return hipGetLastError();
+}
+
 hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
-    const cplx *rhok_ny, const cplx *tw, int Ng, hipStream_t st) {
+    const cplx *slot0, const cplx *tw, int Ng, hipStream_t st) {
   const int Ngh = Ng / 2;
   switch (Ng) {
-#define X(n) case n: return launch_first3<n>(out0, out1, out2, rhok, rhok_ny, tw, Ng, Ngh, st);
+#define X(n) case n: return launch_first3<n>(out0, out1, out2, rhok, slot0, tw, Ng, Ngh, st);
     EZ_FFT_SIZES(X)
 #undef X
     default: return hipErrorInvalidValue;
