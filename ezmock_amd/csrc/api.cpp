@@ -63,7 +63,7 @@ struct Mesh {
   cplx *twN = nullptr, *twh = nullptr;
   real *spec = nullptr;
   // tracer scratch
-  uint32_t *blockcnt = nullptr;
+  uint32_t *blockcnt = nullptr, *blockhist = nullptr;
   uint64_t *blockoff = nullptr, *scan_tmp = nullptr;
   unsigned long long *hist = nullptr, *counter = nullptr;
   rkey *keys = nullptr, *keys_tmp = nullptr;
@@ -159,7 +159,7 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->rho); dfree(&m->ny); dfree(&m->slot0); dfree(&m->rhot); dfree(&m->nt);
   dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
   dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
-  dfree(&m->hist); dfree(&m->counter);
+  dfree(&m->hist); dfree(&m->counter); dfree(&m->blockhist);
   dfree(&m->cic_counts); dfree(&m->cic_list); m->cic_list_cap = 0; dfree(&m->cic_off);
   dfree(&m->cic_cur); dfree(&m->cic_scan);
   free_tracer_scratch(m);
@@ -393,6 +393,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
       (e = dalloc(&m->blockcnt, nblk)) || (e = dalloc(&m->blockoff, nblk)) ||
       (e = dalloc(&m->scan_tmp, scan_tmp_words(nblk) + 8)) ||
       (e = dalloc(&m->hist, EZ_NUM_LOGB_BIN)) || (e = dalloc(&m->counter, 2)) ||
+      (e = dalloc(&m->blockhist, nblk * EZ_NUM_LOGB_BIN)) ||
       (e = dalloc(&m->d_num, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
       (e = dalloc(&m->d_cnt, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
       (e = dalloc(&m->d_vb, EZ_MAX_NUM_TRACER_PER_CELL + 1)) ||
@@ -428,7 +429,8 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   KCHK(launch_zero(m->hist, EZ_NUM_LOGB_BIN * sizeof(unsigned long long), m->st));
   {
     Scope s(m, "bias");
-    KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, bp, m->jtab, m->st));
+    KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, m->blockhist, bp,
+        m->jtab, m->st));
   }
   HIP_OK(hipMemcpyAsync(m->hist_h, m->hist, sizeof(m->hist_h),
       hipMemcpyDeviceToHost, m->st));
@@ -457,7 +459,10 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   KCHK(launch_zero(m->counter, 2 * sizeof(unsigned long long), m->st));
   {
     Scope s(m, "compact");
-    KCHK(launch_compact(m->rhot, rho_thres, N, m->keys, m->keys_cap, m->counter, m->st));
+    KCHK(launch_block_cand(m->blockhist, nblk, imax, m->blockcnt, m->st));
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+    KCHK(launch_compact_at(m->rhot, rho_thres, N, m->blockoff, m->blockcnt,
+        m->keys, m->keys_cap, m->counter, m->st));
   }
   {
     Scope s(m, "sort");
@@ -475,11 +480,13 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   uint64_t res[4] = {0, 0, 0, 0}, ncand = 0;
   std::vector<uint64_t> pcnt(nmax);
   HIP_OK(hipMemcpyAsync(res, m->d_res, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
-  HIP_OK(hipMemcpyAsync(&ncand, m->counter, sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
+  uint64_t cflag = 0;
+  HIP_OK(hipMemcpyAsync(&ncand, m->scan_tmp + nch, sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
+  HIP_OK(hipMemcpyAsync(&cflag, m->counter, sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
   HIP_OK(hipMemcpyAsync(pcnt.data(), m->d_cnt, sizeof(uint64_t) * nmax,
       hipMemcpyDeviceToHost, m->st));
   if ((e = sync_and_collect(m))) return e;
-  if (ncand != cnt || res[2]) return EZMOCK_ERR_UNKNOWN;
+  if (ncand != cnt || cflag || res[2]) return EZMOCK_ERR_UNKNOWN;
   ncell = res[0];
   ntot = res[1];
   m->nmax = nmax;

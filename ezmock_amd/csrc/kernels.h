@@ -67,9 +67,15 @@ hipError_t launch_count_valid(const real *rho, real thres, size_t N,
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
     uint64_t *tmp, hipStream_t st);
 size_t scan_tmp_words(size_t n);
+// blockhist (optional): per-block 64-bin histograms (nblk x 64 u32).
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
-    unsigned long long *hist, const BiasParams &bp, const uint32_t *jtab,
-    hipStream_t st);
+    unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
+    const uint32_t *jtab, hipStream_t st);
+hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
+    uint32_t *blockcnt, hipStream_t st);
+hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
+    const uint64_t *blockoff, const uint32_t *blockcnt, rkey *keys, size_t cap,
+    unsigned long long *flag, hipStream_t st);
 hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
     size_t cap, unsigned long long *counter, hipStream_t st);
 hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,

@@ -161,7 +161,7 @@ __device__ __forceinline__ void gauss_pair(MrgState &g, double sigma,
 __global__ void __launch_bounds__(TPB)
 k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     const uint64_t *__restrict__ blockoff, unsigned long long *__restrict__ hist,
-    BiasParams bp, const uint32_t *__restrict__ jtab) {
+    uint32_t *__restrict__ blockhist, BiasParams bp, const uint32_t *__restrict__ jtab) {
   __shared__ real tile[PADIDX(CPB)];
   __shared__ uint16_t vcell[CPB];       // block-local rank -> cell
   __shared__ uint32_t lh[EZ_NUM_LOGB_BIN];
@@ -241,17 +241,19 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     const size_t i = b0 + c;
     if (i < bp.N) rhot[i] = tile[PADIDX(c)];
   }
-  if (tid < EZ_NUM_LOGB_BIN && lh[tid])
-    atomicAdd(&hist[tid], (unsigned long long) lh[tid]);
+  if (tid < EZ_NUM_LOGB_BIN) {
+    if (lh[tid]) atomicAdd(&hist[tid], (unsigned long long) lh[tid]);
+    if (blockhist) blockhist[(size_t) blockIdx.x * EZ_NUM_LOGB_BIN + tid] = lh[tid];
+  }
 }
 
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
-    unsigned long long *hist, const BiasParams &bp, const uint32_t *jtab,
-    hipStream_t st) {
+    unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
+    const uint32_t *jtab, hipStream_t st) {
   BiasParams b2 = bp;
   mrg_jump(b2.seed_state, (uint64_t) 2 * bp.N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_bias, dim3(ez_blocks(bp.N, CPB)), dim3(TPB), 0, st,
-      rho, rhot, blockoff, hist, b2, jtab);
+      rho, rhot, blockoff, hist, blockhist, b2, jtab);
   return hipGetLastError();
 }
 
@@ -301,6 +303,64 @@ hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
     size_t cap, unsigned long long *counter, hipStream_t st) {
   hipLaunchKernelGGL(k_compact, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
       rhot, thres, N, keys, cap, counter);
+  return hipGetLastError();
+}
+
+// Candidate compaction without a shared counter: the bias kernel's
+// per-block histograms give each block's number of candidates (the cells in
+// log2 bins <= imax, i.e. rhot >= 2^(imax_bias - imax)); their exclusive
+// scan gives every block its output range.
+__global__ void k_block_cand(const uint32_t *__restrict__ blockhist, size_t nblk,
+    int imax, uint32_t *__restrict__ blockcnt) {
+  const size_t b = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  uint32_t c = 0;
+  for (int i = 0; i <= imax; i++) c += blockhist[b * EZ_NUM_LOGB_BIN + i];
+  blockcnt[b] = c;
+}
+
+hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
+    uint32_t *blockcnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_block_cand, dim3(ez_blocks(nblk, 256)), dim3(256), 0, st,
+      blockhist, nblk, imax, blockcnt);
+  return hipGetLastError();
+}
+
+// flag[0] |= 1 if a block's candidates disagree with its histogram count.
+__global__ void __launch_bounds__(TPB)
+k_compact_at(const real *__restrict__ rhot, real thres, size_t N,
+    const uint64_t *__restrict__ blockoff, const uint32_t *__restrict__ blockcnt,
+    rkey *__restrict__ keys, size_t cap, unsigned long long *__restrict__ flag) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  real v[CPT];
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    v[q] = (i < N) ? rhot[i] : (real) 0;
+    if (i < N && v[q] >= thres) c++;
+  }
+  uint32_t tot;
+  const uint32_t excl = block_excl_scan<uint32_t>(c, tot, ws);
+  if (threadIdx.x == 0 && tot != blockcnt[blockIdx.x]) atomicOr(flag, 1ull);
+  if (!c) return;
+  size_t pos = blockoff[blockIdx.x] + excl;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    if (i < N && v[q] >= thres) {
+      if (pos < cap) keys[pos] = real_key(v[q]);
+      pos++;
+    }
+  }
+}
+
+hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
+    const uint64_t *blockoff, const uint32_t *blockcnt, rkey *keys, size_t cap,
+    unsigned long long *flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact_at, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
+      rhot, thres, N, blockoff, blockcnt, keys, cap, flag);
   return hipGetLastError();
 }
 
