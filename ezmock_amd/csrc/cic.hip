@@ -341,24 +341,24 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
     const real wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
     real qv[3] = {0, 0, 0};
     if (MOM) { qv[0] = qq[u][0]; qv[1] = qq[u][1]; qv[2] = qq[u][2]; }
+    // the eight corners, in an order rotated by lane: lanes whose particles
+    // share a cell (dense regions) then add to different cells in the same
+    // LDS instruction instead of serialising on one address
+    const int rot = (int) (threadIdx.x & 7);
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
-      if (lx[a] < 0) continue;
+    for (int ci = 0; ci < 8; ci++) {
+      const int cc = (ci + rot) & 7;
+      const int a = cc >> 2, b = (cc >> 1) & 1, d = cc & 1;
+      const int la = a ? lx[1] : lx[0], lb = b ? ly[1] : ly[0], ld = d ? lz[1] : lz[0];
+      if (la < 0 || lb < 0 || ld < 0) continue;
+      const real xa = a ? wx[1] : wx[0], yb = b ? wy[1] : wy[0], zd = d ? wz[1] : wz[0];
+      const int li = (la * TYY + lb) * TZZ + ld;
+      atomicAdd(&acc[0][li], (double) (xa * yb * zd));
+      if (MOM) {
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        if (ly[b] < 0) continue;
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-          if (lz[d] < 0) continue;
-          const int li = (lx[a] * TYY + ly[b]) * TZZ + lz[d];
-          atomicAdd(&acc[0][li], (double) (wx[a] * wy[b] * wz[d]));
-          if (MOM) {
-#pragma unroll
-            for (int f = 0; f < 3; f++) {
-              const real u = wx[a] * qv[f];          // pop_tracer.c:641-642
-              atomicAdd(&acc[1 + f][li], (double) (u * wy[b] * wz[d]));
-            }
-          }
+        for (int f = 0; f < 3; f++) {
+          const real u = xa * qv[f];                // pop_tracer.c:641-642
+          atomicAdd(&acc[1 + f][li], (double) (u * yb * zd));
         }
       }
     }
