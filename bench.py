@@ -97,6 +97,39 @@ def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=T
     return dt, ntr, kt, lat
 
 
+def run_large(Ng=1536, L=3000.0):
+    """Large single mock: 1536^3, Lbox 3000 Mpc/h, z 1.0, seed 7, 5e-4 (h/Mpc)^3
+    tracers (bias (0.5, 10, 0.3, 150)), FP32; one warm-up and one timed mock,
+    with the algorithmic GB/s of the FFT passes, the CIC and the bias kernel."""
+    from ezmock_amd import EZmock, inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=7, precision="f32")
+    ez.eval_growth_params(1.0, 0.31)
+    ez.setup_linear_pk(k, P)
+    nexp = int(round(5e-4 * L ** 3))
+    ez.create_dens_field_from_ic()
+    ntr = ez.populate_tracer_device(0.5, 10.0, 0.3, 150.0, nexp)
+    ez.profile(True)
+    ez.reset_kernel_times()
+    ez.sync()
+    t0 = time.perf_counter()
+    ez.create_dens_field_from_ic()
+    ntr = ez.populate_tracer_device(0.5, 10.0, 0.3, 150.0, nexp)
+    ez.sync()
+    dt = time.perf_counter() - t0
+    kt = ez.kernel_times()
+    gbs = {}
+    for name in ("fft_pass_i", "fft_pass_j", "fft_rows_c2r", "cic_bin_count", "cic_bin_scatter",
+                 "cic_tiles_rho_mom", "bias"):
+        if name in kt:
+            t, c = kt[name]
+            b = algorithmic_bytes(name, Ng, 4, ntr)
+            gbs[name] = round(b * c / (t * 1e-3) / 1e9, 1)
+    del ez
+    return {"ms_per_mock": round(dt * 1e3, 1), "tracers": ntr,
+            "alg_GBps": gbs, "peak_GBps": HBM_PEAK_GBS}
+
+
 def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
     """Calibration sweep: nmock mocks at Ng^3 with seeds 1..nmock and bias
     parameters drawn from a fixed meta-seed (rho_c in [0.2,1.2], rho_exp in
@@ -259,6 +292,11 @@ def main():
             log(f"bench: 512^3 f64 {extra['ms_per_mock_512_f64']} ms/mock")
         except Exception as e:
             extra["ms_per_mock_512_f64"] = f"failed: {e}"
+        try:
+            extra["large_1536_f32"] = run_large()
+            log(f"bench: 1536^3 {extra['large_1536_f32']}")
+        except Exception as e:
+            extra["large_1536_f32"] = f"failed: {e}"
         try:
             extra["batch_64x256"] = run_batch()
             log(f"bench: batch {extra['batch_64x256']}")

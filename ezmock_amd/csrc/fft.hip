@@ -554,10 +554,19 @@ static hipError_t launch_first3_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *rho
   return hipGetLastError();
 }
 
+// The three-component pass keeps its tile in registers and runs three
+// transforms per workgroup: the narrowest power-of-two tile whose LDS
+// reaches 64 KB (at most 16 columns; measured at 512, 1024 and 1536).
+template <int L> struct First3Cfg {
+  static constexpr int bytes = L * (int) sizeof(cplx);
+  static constexpr int TKMAX = (16 * bytes <= 65536) ? 16 : (2 * bytes >= 65536) ? 2
+      : (4 * bytes >= 65536) ? 4 : 8;
+};
+
 template <int L>
 static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
     const cplx *slot0, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
-  constexpr int T = StridedCfg<L>::TKMAX;
+  constexpr int T = First3Cfg<L>::TKMAX;
   if (Ngh % T == 0) return launch_first3_tk<L, T>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
   if constexpr (T >= 8) if (Ngh % (T / 2) == 0) return launch_first3_tk<L, T / 2>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
   if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_first3_tk<L, 4>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
