@@ -311,12 +311,13 @@ def device_count(precision="f32"):
     return _lib.load(precision).EZmock_ext_device_count()
 
 
-def uniform(seed, start, n, rng=0, precision="f32"):
-    """GPU draws of stream `rng` (reset(seed, start), then n get_double)."""
+def uniform(seed, start, n, rng=0, precision="f32", pos=False):
+    """GPU draws of stream `rng` (reset(seed, start), then n get_double, or
+    get_double_pos with pos=True)."""
     L = _lib.load(precision)
     out = np.empty(int(n), dtype=np.float64)
     err = C.c_int(0)
-    L.EZmock_ext_uniform(int(rng), int(seed), int(start), int(n), False, out.ctypes.data,
+    L.EZmock_ext_uniform(int(rng), int(seed), int(start), int(n), bool(pos), out.ctypes.data,
                          C.byref(err))
     if err.value:
         raise RuntimeError(L.EZmock_errmsg(err.value).decode())

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "common.h"
+#include "mt.h"
 #include "rng.h"
 #include "fft.h"
 #include "kernels.h"
@@ -1258,20 +1259,31 @@ int EZmock_ext_sync(EZMOCK *ez, int *err) {
 int EZmock_ext_uniform(const int randgen, const uint64_t seed,
     const uint64_t start, const size_t n, const bool pos, double *out,
     int *err) {
-  (void) pos;                   // MRG32k3a: get_double == get_double_pos
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err) return *err;
-  if (randgen != 0) return (*err = EZMOCK_ERR_RNG_SET);
+  if (randgen != 0 && randgen != 1) return (*err = EZMOCK_ERR_ARG_RNG);
   Mesh tmp;
   int e = device_ready(&tmp);
   if (e) return (*err = e);
   double *d = nullptr;
   if ((e = dalloc(&d, n))) { free_mesh_device(&tmp); return (*err = e); }
-  MrgState s;
-  mrg_seed_state(seed, s);
-  if (launch_uniform(s, start, n, tmp.jtab, d, tmp.st) != hipSuccess ||
-      hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
-      hipStreamSynchronize(tmp.st) != hipSuccess)
+  if (randgen == 0) {           // MRG32k3a: get_double == get_double_pos
+    MrgState s;
+    mrg_seed_state(seed, s);
+    if (launch_uniform(s, start, n, tmp.jtab, d, tmp.st) != hipSuccess) e = EZMOCK_ERR_UNKNOWN;
+  }
+  else {                        // MT19937: two output words per draw
+    uint32_t *w = nullptr;
+    if ((e = dalloc(&w, 2 * n)) == 0) {
+      if (mt_stream_words(seed, 2 * start, 2 * n, w, tmp.st) != hipSuccess ||
+          launch_uniform_mt(w, n, pos ? 1 : 0, d, tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_RNG_JUMP;
+      if (hipStreamSynchronize(tmp.st) != hipSuccess) e = EZMOCK_ERR_UNKNOWN;
+      dfree(&w);
+    }
+  }
+  if (!e && (hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
+      hipStreamSynchronize(tmp.st) != hipSuccess))
     e = EZMOCK_ERR_UNKNOWN;
   dfree(&d);
   free_mesh_device(&tmp);

@@ -94,6 +94,8 @@ hipError_t launch_velocities(const real *x, const real *y, const real *z,
     hipStream_t st);
 hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
     const uint32_t *jtab, double *out, hipStream_t st);
+hipError_t launch_uniform_mt(const uint32_t *w, size_t n, int pos, double *out,
+    hipStream_t st);
 hipError_t launch_rsd(const real *z, const real *vz, size_t n, double fac,
     double Lbox, double *out, hipStream_t st);
 

@@ -1,0 +1,220 @@
+// mt.hip -- MT19937 output streams on gfx950.
+//
+// A stream segment of words [first, first + n) is produced in three steps:
+//   1. the seed window is moved to word `first` with one jump (x^first mod
+//      phi, from mt_host.cpp);
+//   2. segment start windows every J words are built level by level: each
+//      workgroup of a level walks a chain of 16 jumps of the level's stride
+//      (16^k J), so S segments need 15 log16(S) jumps in sequence;
+//   3. one workgroup per segment runs the twist recurrence in LDS and writes
+//      its J tempered words.
+// A jump applies g(A) = sum_i g_i A^i to a window with a blocked Horner
+// scheme: acc <- A^B acc + sum_{r<B} g_{qB+r} A^r W over blocks q of B = 224
+// coefficients; A^r W is the input window shifted by r words, so each block
+// is a word-wise XOR over the set coefficients, and A^B acc is one parallel
+// generation of B words (B <= 227 keeps every new word a function of the
+// old window only).
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+#include "mt.h"
+
+#define MT_N 624
+#define MT_M 397
+#define MT_TPB 256
+#define MT_B 224
+#define MT_SEG_WORDS (1ull << 18)
+#define MT_FAN 16
+
+__device__ __forceinline__ uint32_t mt_f(uint32_t x0, uint32_t x1, uint32_t xm) {
+  const uint32_t y = (x0 & 0x80000000u) | (x1 & 0x7fffffffu);
+  return xm ^ (y >> 1) ^ ((x1 & 1u) ? 0x9908b0dfu : 0u);
+}
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+  y ^= (y >> 11);
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= (y >> 18);
+  return y;
+}
+
+// One workgroup per chain: out[c F + 0] = in[c], out[c F + k] = g(A) out[c F + k - 1];
+// chains stop at nout windows in total.
+__global__ void __launch_bounds__(MT_TPB)
+k_mt_chain(const uint32_t *__restrict__ g, const uint32_t *__restrict__ in,
+    int fan, size_t nout, uint32_t *__restrict__ out) {
+  __shared__ uint32_t gl[MT_JUMP_WORDS];
+  __shared__ uint32_t S[MT_N + MT_B];
+  __shared__ uint32_t acc[MT_N], nw[MT_B];
+  const int tid = threadIdx.x;
+  const size_t c = blockIdx.x;
+  for (int i = tid; i < MT_JUMP_WORDS; i += MT_TPB) gl[i] = g[i];
+  for (int i = tid; i < MT_N; i += MT_TPB) S[i] = in[c * MT_N + i];
+  __syncthreads();
+  int qtop = 0;                          // highest block with a set coefficient
+  for (int i = MT_JUMP_WORDS - 1; i >= 0; i--)
+    if (gl[i]) { qtop = (i * 32) / MT_B; break; }
+  for (int k = 0; k < fan; k++) {
+    const size_t o = c * fan + k;
+    if (o >= nout) break;
+    if (k > 0) {
+      // S[624 .. 624 + B - 2] = next words of the input window
+      for (int j = tid; j < MT_B - 1; j += MT_TPB)
+        S[MT_N + j] = mt_f(S[j], S[j + 1], S[j + MT_M]);
+      for (int i = tid; i < MT_N; i += MT_TPB) acc[i] = 0;
+      __syncthreads();
+      for (int q = qtop; q >= 0; q--) {
+        // acc <- A^B acc
+        if (q != qtop) {
+          for (int j = tid; j < MT_B; j += MT_TPB) nw[j] = mt_f(acc[j], acc[j + 1], acc[j + MT_M]);
+          __syncthreads();
+          uint32_t keep[(MT_N + MT_TPB - 1) / MT_TPB];
+#pragma unroll
+          for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+            const int i = tid + u * MT_TPB;
+            if (i < MT_N) keep[u] = (i + MT_B < MT_N) ? acc[i + MT_B] : nw[i + MT_B - MT_N];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+            const int i = tid + u * MT_TPB;
+            if (i < MT_N) acc[i] = keep[u];
+          }
+        }
+        // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words)
+#pragma unroll
+        for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+          const int i = tid + u * MT_TPB;
+          if (i >= MT_N) continue;
+          uint32_t t = 0;
+          for (int wq = 0; wq < MT_B / 32; wq++) {
+            const int gw = q * (MT_B / 32) + wq;
+            uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
+            while (bits) {
+              const int b = __ffs(bits) - 1;
+              bits &= bits - 1;
+              t ^= S[i + wq * 32 + b];
+            }
+          }
+          acc[i] ^= t;
+        }
+        __syncthreads();
+      }
+      for (int i = tid; i < MT_N; i += MT_TPB) S[i] = acc[i];
+      __syncthreads();
+    }
+    for (int i = tid; i < MT_N; i += MT_TPB) out[o * MT_N + i] = S[i];
+    __syncthreads();
+  }
+}
+
+// One workgroup per segment s: words [s J, min((s + 1) J, n)) of the stream
+// whose window at word s J is starts[s].
+__global__ void __launch_bounds__(MT_TPB)
+k_mt_fill(const uint32_t *__restrict__ starts, uint64_t J, uint64_t n,
+    uint32_t *__restrict__ out) {
+  __shared__ uint32_t a[MT_N], b[MT_N];
+  const int tid = threadIdx.x;
+  const uint64_t s = blockIdx.x;
+  for (int i = tid; i < MT_N; i += MT_TPB) a[i] = starts[s * MT_N + i];
+  __syncthreads();
+  const uint64_t w0 = s * J;
+  const uint64_t w1 = (w0 + J < n) ? w0 + J : n;
+  uint32_t *cur = a, *nxt = b;
+  for (uint64_t w = w0; w < w1; w += MT_N) {
+    // new[j] = f(x_j, x_{j+1}, x_{j+397}) in three dependent phases
+    for (int j = tid; j < MT_N - MT_M; j += MT_TPB)
+      nxt[j] = mt_f(cur[j], cur[j + 1], cur[j + MT_M]);
+    __syncthreads();
+    for (int j = MT_N - MT_M + tid; j < 2 * (MT_N - MT_M); j += MT_TPB)
+      nxt[j] = mt_f(cur[j], cur[j + 1], nxt[j + MT_M - MT_N]);
+    __syncthreads();
+    for (int j = 2 * (MT_N - MT_M) + tid; j < MT_N; j += MT_TPB)
+      nxt[j] = mt_f(cur[j], (j + 1 < MT_N) ? cur[j + 1] : nxt[0], nxt[j + MT_M - MT_N]);
+    __syncthreads();
+    for (int j = tid; j < MT_N; j += MT_TPB)
+      if (w + j < w1) out[w + j] = mt_temper(nxt[j]);
+    uint32_t *t = cur; cur = nxt; nxt = t;
+    __syncthreads();
+  }
+}
+
+namespace {
+std::mutex g_mu;
+std::map<std::pair<int, uint64_t>, uint32_t *> g_dev_poly;   // (device, e) -> g
+
+const uint32_t *device_poly(uint64_t e) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_dev_poly.find({dev, e});
+    if (it != g_dev_poly.end()) return it->second;
+  }
+  const uint32_t *h = mt_jump_poly(e);
+  if (!h) return nullptr;
+  uint32_t *d = nullptr;
+  if (hipMalloc(&d, MT_JUMP_WORDS * sizeof(uint32_t)) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h, MT_JUMP_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    (void) hipFree(d);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_dev_poly[{dev, e}] = d;
+  return d;
+}
+}  // namespace
+
+hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
+    uint32_t *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const uint64_t J = MT_SEG_WORDS;
+  const uint64_t nseg = (n + J - 1) / J;
+  int L = 0;
+  uint64_t cap = 1;
+  while (cap < nseg) { cap *= MT_FAN; L++; }
+  uint32_t w0[MT_N];
+  mt_seed_window(seed, w0);
+  // window buffers: level windows ping-pong, sized for nseg (+ fan slack)
+  const size_t maxw = (size_t) nseg + MT_FAN;
+  uint32_t *wa = nullptr, *wb = nullptr;
+  hipError_t e;
+  if ((e = hipMallocAsync((void **) &wa, maxw * MT_N * sizeof(uint32_t), st)) != hipSuccess) return e;
+  if ((e = hipMallocAsync((void **) &wb, maxw * MT_N * sizeof(uint32_t), st)) != hipSuccess) {
+    (void) hipFreeAsync(wa, st);
+    return e;
+  }
+  auto fail = [&](hipError_t er) { (void) hipFreeAsync(wa, st); (void) hipFreeAsync(wb, st); return er; };
+  if ((e = hipMemcpyAsync(wa, w0, sizeof(w0), hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);  // w0 is on the host stack
+  if (first) {
+    const uint32_t *g = device_poly(first);
+    if (!g) return fail(hipErrorInvalidValue);
+    hipLaunchKernelGGL(k_mt_chain, dim3(1), dim3(MT_TPB), 0, st, g, wa, 2, (size_t) 2, wb);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(wa, wb + MT_N, MT_N * sizeof(uint32_t), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+      return fail(e);
+  }
+  // level l: count windows at stride J * FAN^(L-1-l) from count/FAN at the coarser one
+  uint64_t have = 1;
+  for (int l = 0; l < L; l++) {
+    uint64_t stride = J;
+    for (int k = 0; k < L - 1 - l; k++) stride *= MT_FAN;
+    const uint64_t need = (nseg + stride / J - 1) / (stride / J);
+    const uint32_t *g = device_poly(stride);
+    if (!g) return fail(hipErrorInvalidValue);
+    hipLaunchKernelGGL(k_mt_chain, dim3((unsigned) have), dim3(MT_TPB), 0, st, g, wa,
+        MT_FAN, (size_t) need, wb);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    uint32_t *t = wa; wa = wb; wb = t;
+    have = need;
+  }
+  hipLaunchKernelGGL(k_mt_fill, dim3((unsigned) nseg), dim3(MT_TPB), 0, st, wa, J, n, out);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+  (void) hipFreeAsync(wa, st);
+  (void) hipFreeAsync(wb, st);
+  return hipSuccess;
+}

@@ -79,3 +79,14 @@ __host__ __device__ __forceinline__ void mrg_jump(MrgState &g, uint64_t n,
 // and the jump table.
 void mrg_seed_state(uint64_t seed, MrgState &g);
 const uint32_t *mrg_host_jump_table();
+
+// MT19937 draws from a buffer of tempered output words (two words per draw,
+// genrand_res53): get_double in [0, 1), get_double_pos in (0, 1).
+__host__ __device__ __forceinline__ double mt_draw(const uint32_t *w, uint64_t d) {
+  const uint32_t a = w[2 * d] >> 5, b = w[2 * d + 1] >> 6;
+  return ((double) a * 67108864.0 + (double) b) * (1.0 / 9007199254740992.0);
+}
+__host__ __device__ __forceinline__ double mt_draw_pos(const uint32_t *w, uint64_t d) {
+  const uint32_t a = w[2 * d] >> 5, b = w[2 * d + 1] >> 6;
+  return ((double) a * 67108864.0 + (double) b + 0.5) * (1.0 / 9007199254740992.0);
+}

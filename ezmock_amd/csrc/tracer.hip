@@ -761,6 +761,19 @@ __global__ void k_uniform(MrgState seed, uint64_t start, size_t n,
   for (size_t i = i0; i < i0 + 64 && i < n; i++) out[i] = mrg_next(g);
 }
 
+__global__ void k_uniform_mt(const uint32_t *__restrict__ w, size_t n, int pos,
+    double *__restrict__ out) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pos ? mt_draw_pos(w, i) : mt_draw(w, i);
+}
+
+hipError_t launch_uniform_mt(const uint32_t *w, size_t n, int pos, double *out,
+    hipStream_t st) {
+  hipLaunchKernelGGL(k_uniform_mt, dim3(ez_blocks(n, 256)), dim3(256), 0, st, w, n,
+      pos, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
     const uint32_t *jtab, double *out, hipStream_t st) {
   const size_t nthr = (n + 63) / 64;

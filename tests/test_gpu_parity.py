@@ -25,6 +25,31 @@ def test_uniform_draws_bit_exact(gpu_available):
             assert np.array_equal(g, o), (seed, start)
 
 
+@pytest.mark.parametrize("seed", [1, 42, 5489, 2 ** 40 + 3])
+def test_uniform_draws_mt19937_bit_exact(gpu_available, seed):
+    """MT19937 streams on the GPU (jump-ahead by x^e mod phi, segment
+    generation) against the oracle's sequential generator: get_double and
+    get_double_pos, at offsets below and across one segment (2^18 words)."""
+    for start in (0, 1, 311, 312, 997, 131071, 131072, 300001):
+        o = po.Rng(1, seed, start, prec="f64")
+        ref = o.double(700)
+        assert np.array_equal(uniform(seed, start, 700, rng=1), ref), (seed, start)
+        o = po.Rng(1, seed, start, prec="f64")
+        ref = o.double_pos(700)
+        assert np.array_equal(uniform(seed, start, 700, rng=1, pos=True), ref), (seed, start)
+
+
+def test_mt19937_long_stream_segments(gpu_available):
+    """A stream long enough for two levels of segment windows (> 16 x 2^18
+    words): spot checks against the oracle at the segment boundaries."""
+    n = 2_200_000                                   # 4.4 M words, 17 segments
+    g = uniform(7, 0, n, rng=1)
+    for d in (0, 131071, 131072, 131073, 2 * 131072, 16 * 131072 - 1, 16 * 131072,
+              16 * 131072 + 5, n - 3):
+        ref = po.Rng(1, 7, d, prec="f64").double(3)
+        assert np.array_equal(g[d:d + 3], ref[:len(g[d:d + 3])]), d
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("Ng", [8, 12, 16, 24, 32, 48, 64, 96])
 def test_fft_against_numpy(gpu_available, Ng, prec):
