@@ -77,6 +77,9 @@ struct Mesh {
   size_t rsd_cap = 0;
   // power-spectrum estimator: two assignment grids, a Nyquist plane, sums
   real *pk_grid[2] = {nullptr, nullptr};
+  // MT19937 stream words of the current stage (two per draw)
+  uint32_t *mtw = nullptr;
+  size_t mtw_cap = 0;
   cplx *pk_ny = nullptr;
   double *pk_acc = nullptr;
   real *cat[6] = {nullptr};
@@ -167,6 +170,7 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
   dfree(&m->pk_grid[0]); dfree(&m->pk_grid[1]); dfree(&m->pk_ny); dfree(&m->pk_acc);
+  dfree(&m->mtw); m->mtw_cap = 0;
   for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
   m->cat_cap = 0;
   for (auto &t : m->pending) { (void) hipEventDestroy(t.a); (void) hipEventDestroy(t.b); }
@@ -316,6 +320,25 @@ int r2c_field(Mesh *m, int Ng, real *field) {
   return 0;
 }
 
+// MT19937 instances: stream words of draws [first, first + n) of the
+// instance's seed into m->mtw (nullptr for MRG32k3a instances).
+int mt_words(EZMOCK *ez, Mesh *m, uint64_t first, uint64_t n, const uint32_t **out) {
+  *out = nullptr;
+  if (RNG(ez)->type != 1 || n == 0) return 0;
+  if (m->mtw_cap < 2 * n) {
+    dfree(&m->mtw);
+    m->mtw_cap = 0;
+    int e = dalloc(&m->mtw, 2 * n);
+    if (e) return e;
+    m->mtw_cap = 2 * n;
+  }
+  Scope s(m, "mt_stream");
+  if (mt_stream_words(RNG(ez)->seed, 2 * first, 2 * n, m->mtw, m->st) != hipSuccess)
+    return EZMOCK_ERR_RNG_JUMP;
+  *out = m->mtw;
+  return 0;
+}
+
 // pop_tracer.c:94-133
 int eval_pdf(size_t nexp, real pbase, size_t ncells, std::vector<size_t> &num,
     size_t &ncell, size_t &ntot) {
@@ -428,10 +451,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if ((e = sync_and_collect(m))) return e;
   if (nvalid < ncell) return EZMOCK_ERR_RHO_LESS;
   KCHK(launch_zero(m->hist, EZ_NUM_LOGB_BIN * sizeof(unsigned long long), m->st));
+  const uint32_t *mtw_bias = nullptr;
+  if ((e = mt_words(ez, m, 2 * (uint64_t) N, 2 * ((nvalid + 1) / 2), &mtw_bias))) return e;
   {
     Scope s(m, "bias");
     KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, m->blockhist, bp,
-        m->jtab, m->st));
+        m->jtab, mtw_bias, m->st));
   }
   HIP_OK(hipMemcpyAsync(m->hist_h, m->hist, sizeof(m->hist_h),
       hipMemcpyDeviceToHost, m->st));
@@ -507,10 +532,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if ((e = ensure_cat(m, ntot))) return e;
   const real gs = conf->Lbox / conf->Ng;
   const real igs = conf->Ng / conf->Lbox;
+  const uint32_t *mtw_pos = nullptr;
+  if ((e = mt_words(ez, m, 5 * (uint64_t) N, 3 * (uint64_t) ntot, &mtw_pos))) return e;
   {
     Scope s(m, "positions");
     KCHK(launch_positions(m->nt, m->blockoff, Ng, conf->Lbox, gs, bp.seed_state,
-        m->jtab, m->cat[0], m->cat[1], m->cat[2], ntot, m->st));
+        m->jtab, mtw_pos, m->cat[0], m->cat[1], m->cat[2], ntot, m->st));
   }
   // ---- velocities, pop_tracer.c:593-753 and :1043-1113
   if (!m->mom_valid) {
@@ -519,10 +546,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     if ((e = paint_cic(m, Ng, igs, nullptr, true))) return e;
     m->mom_valid = !m->psi_ref;
   }
+  const uint32_t *mtw_vel = nullptr;
+  if ((e = mt_words(ez, m, 8 * (uint64_t) N, 6 * (((uint64_t) ntot + 1) / 2), &mtw_vel))) return e;
   {
     Scope s(m, "velocities");
     KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, Ng, igs, cosmo->vfac, (double) sigma_v,
-        bp.seed_state, m->jtab, m->cat[3], m->cat[4], m->cat[5], m->st));
+        bp.seed_state, m->jtab, mtw_vel, m->cat[3], m->cat[4], m->cat[5], m->st));
   }
   if ((e = sync_and_collect(m))) return e;
   m->ntracer = ntot;
@@ -770,11 +799,6 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
           "(supported: 2^n and 3*2^n, 8..2048)\n", Ng);
       return (*err = EZMOCK_ERR_ARG_NGRID);
     }
-    if (!delta && RNG(ez)->type != 0) {
-      fprintf(stderr, "EZmock: MT19937 streams are not available on the GPU "
-          "engine; use randgen = 0 (MRG32k3a)\n");
-      return (*err = EZMOCK_ERR_RNG_SET);
-    }
   }
   if ((e = device_ready(m))) return (*err = e);
   m->have_field = false;
@@ -846,8 +870,10 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
           m->ny, gp, m->ptab, spec, m->st));
     }
     else {
+      const uint32_t *mtw = nullptr;
+      if ((e = mt_words(ez, m, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw))) return (*err = e);
       Scope s(m, "gen_spectrum");
-      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, m->st));
+      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->st));
     }
     // the generated Nyquist plane lives in ny[plane, 2 plane)
     cplx *ny_save = m->ny;

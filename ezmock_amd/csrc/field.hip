@@ -53,7 +53,8 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 // One thread = CH consecutive cells of the half-complex grid (reference
 // order, k fastest, Ngk = Ng/2 + 1 per row; Ng even).  Cell idx >= 1
 // consumes draws (idx-1)*2 (amplitude, then phase) or (idx-1) with fixed
-// amplitudes.  Output: rhok[i][j][kk < Ng/2] and rhok_ny[i][j] (the
+// amplitudes: MRG32k3a by jump-ahead per thread, MT19937 (MT) from the
+// stream words generated beforehand (mtw, two words per draw).  Output: rhok[i][j][kk < Ng/2] and rhok_ny[i][j] (the
 // kz = Ng/2 plane); optional copy in the reference layout.
 // The block's 256 CH cells map to one contiguous range of rhok (minus the
 // Nyquist cells): values are staged in LDS and stored by consecutive lanes
@@ -62,11 +63,11 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 #define GEN_TPB 256
 #define GEN_SKIP 0xffffu
 
-template <int CH>
+template <int CH, bool MT>
 __global__ void __launch_bounds__(GEN_TPB)
 k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
     real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
-    const uint32_t *__restrict__ jtab) {
+    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw) {
   constexpr int NB = GEN_TPB * CH;
   __shared__ cplx sv[NB + NB / CH];             // one pad slot per thread
   __shared__ uint16_t so[NB];
@@ -80,10 +81,9 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   int i = (int) (row / (size_t) Ng);
   int j = (int) (row - (size_t) i * Ng);
   MrgState g = gp.seed_state;
-  if (idx0 < gp.Nh) {
-    const uint64_t mult = gp.fixamp ? 1 : 2;
-    mrg_jump(g, (idx0 == 0 ? 0 : (idx0 - 1)) * mult, jtab);
-  }
+  const uint64_t mult = gp.fixamp ? 1 : 2;
+  uint64_t d = (idx0 == 0 ? 0 : (idx0 - 1)) * mult;   // next draw (MT19937 words)
+  if (!MT && idx0 < gp.Nh) mrg_jump(g, d, jtab);
   for (int c = 0; c < CH; c++) {
     const size_t idx = idx0 + c;
     const int e = threadIdx.x * CH + c;
@@ -93,8 +93,15 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
       bool write = true, mirror = false;
       int ni = 0, nj = 0;
       if (idx != 0) {
-        const double amp = gp.fixamp ? 1.0 : mrg_next(g);
-        double ph = mrg_next(g) * 2 * EZ_PI;
+        double amp, ph;
+        if (MT) {
+          amp = gp.fixamp ? 1.0 : mt_draw_pos(mtw, d++);
+          ph = mt_draw(mtw, d++) * 2 * EZ_PI;
+        }
+        else {
+          amp = gp.fixamp ? 1.0 : mrg_next(g);
+          ph = mrg_next(g) * 2 * EZ_PI;
+        }
         const bool plane_k = (k == 0) || (k == Ngh);
         if (plane_k) {
           ni = (i == 0) ? 0 : Ng - i;
@@ -156,12 +163,16 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
 
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    hipStream_t st) {
+    const uint32_t *mtw, hipStream_t st) {
   if (gp.Ng & 1) return hipErrorInvalidValue;
   constexpr int CH = sizeof(real) == 4 ? 16 : 8;
   const size_t nthr = (gp.Nh + CH - 1) / CH;
-  hipLaunchKernelGGL(k_gen_rhok<CH>, dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
-      st, rhok, rhok_ny, ref_copy, gp, ptab, jtab);
+  if (mtw)
+    hipLaunchKernelGGL((k_gen_rhok<CH, true>), dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
+        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw);
+  else
+    hipLaunchKernelGGL((k_gen_rhok<CH, false>), dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
+        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw);
   return hipGetLastError();
 }
 

@@ -15,9 +15,10 @@ struct GenParams {
 hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
     const double *Pt, const double *P2t, int n, int logpk, double kfac,
     double logkfac, hipStream_t st);
+// mtw: MT19937 stream words of draws 0 .. (mult (Nh - 1)) (nullptr: MRG32k3a)
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    hipStream_t st);
+    const uint32_t *mtw, hipStream_t st);
 hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     const cplx *dk_ny, const GenParams &gp, const double *ptab,
     real *ref_copy, hipStream_t st);
@@ -68,9 +69,10 @@ hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
     uint64_t *tmp, hipStream_t st);
 size_t scan_tmp_words(size_t n);
 // blockhist (optional): per-block 64-bin histograms (nblk x 64 u32).
+// mtw (MT19937, else nullptr): stream words of draws 2N + 2 p, pair p.
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
     unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
-    const uint32_t *jtab, hipStream_t st);
+    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st);
 hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
     uint32_t *blockcnt, hipStream_t st);
 hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
@@ -85,13 +87,15 @@ hipError_t launch_walk(const rkey *sorted, size_t M, const uint64_t *num,
     int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st);
 hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
     int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st);
+// mtw (MT19937, else nullptr): stream words of draws 5N + 3 t, tracer t.
 hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
-    real *x, real *y, real *z, size_t ntot, hipStream_t st);
+    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st);
+// mtw (MT19937, else nullptr): stream words of draws 8N + 6 q, pair q.
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
     const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
-    MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
-    hipStream_t st);
+    MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
+    real *vz, hipStream_t st);
 hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
     const uint32_t *jtab, double *out, hipStream_t st);
 hipError_t launch_uniform_mt(const uint32_t *w, size_t n, int pos, double *out,

@@ -145,11 +145,22 @@ __device__ __forceinline__ int hist_bin(real v, int imax) {
   return d;
 }
 
+// Sequential draws from one stream position on: an MRG32k3a state, or
+// MT19937 output words generated beforehand (w != nullptr, two words per
+// draw, d = the draw's index in w).
+struct Draws {
+  MrgState g;
+  const uint32_t *w;
+  uint64_t d;
+  __device__ __forceinline__ double next() { return w ? mt_draw(w, d++) : mrg_next(g); }
+  __device__ __forceinline__ double next_pos() { return w ? mt_draw_pos(w, d++) : mrg_next(g); }
+};
+
 // pop_tracer.c:70-79
-__device__ __forceinline__ void gauss_pair(MrgState &g, double sigma,
+__device__ __forceinline__ void gauss_pair(Draws &s, double sigma,
     double &g1, double &g2) {
-  const double r = -2 * log(mrg_next(g));
-  const double th = mrg_next(g) * 2 * EZ_PI;
+  const double r = -2 * log(s.next_pos());
+  const double th = s.next() * 2 * EZ_PI;
   g1 = sigma * sqrt(r) * cos(th);
   g2 = sigma * sqrt(r) * sin(th);
 }
@@ -161,7 +172,8 @@ __device__ __forceinline__ void gauss_pair(MrgState &g, double sigma,
 __global__ void __launch_bounds__(TPB)
 k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     const uint64_t *__restrict__ blockoff, unsigned long long *__restrict__ hist,
-    uint32_t *__restrict__ blockhist, BiasParams bp, const uint32_t *__restrict__ jtab) {
+    uint32_t *__restrict__ blockhist, BiasParams bp, const uint32_t *__restrict__ jtab,
+    const uint32_t *__restrict__ mtw) {
   __shared__ real tile[PADIDX(CPB)];
   __shared__ uint16_t vcell[CPB];       // block-local rank -> cell
   __shared__ uint32_t lh[EZ_NUM_LOGB_BIN];
@@ -202,7 +214,7 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
   // pairs pb .. pe cover global ranks rb .. rb + tot - 1
   const uint64_t npair = tot ? ((rb + tot - 1) >> 1) - pb + 1 : 0;
   const uint64_t ppt = (npair + TPB - 1) / TPB;
-  if (tid == 0 && npair) {
+  if (tid == 0 && npair && !mtw) {
     MrgState g = bp.seed_state;            // advanced to draw 2N
     mrg_jump(g, 2 * pb, jtab);
     base = g;
@@ -212,13 +224,18 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
   // one Box-Muller pair per iteration on every lane
   const uint64_t m0 = (uint64_t) tid * ppt;
   if (m0 < npair) {
-    MrgState g = base;
-    mrg_jump(g, 2 * m0, jtab);
+    Draws ds;
+    ds.w = mtw;
+    ds.d = 2 * (pb + m0);                  // MT19937 words start at draw 2N
+    if (!mtw) {
+      ds.g = base;
+      mrg_jump(ds.g, 2 * m0, jtab);
+    }
     const uint64_t m1 = (m0 + ppt < npair) ? m0 + ppt : npair;
 #pragma unroll 1
     for (uint64_t m = m0; m < m1; m++) {
       double gg[2];
-      gauss_pair(g, bp.lambda, gg[0], gg[1]);
+      gauss_pair(ds, bp.lambda, gg[0], gg[1]);
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const uint64_t r = 2 * (pb + m) + h;          // global rank
@@ -249,11 +266,11 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
 
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
     unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
-    const uint32_t *jtab, hipStream_t st) {
+    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st) {
   BiasParams b2 = bp;
   mrg_jump(b2.seed_state, (uint64_t) 2 * bp.N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_bias, dim3(ez_blocks(bp.N, CPB)), dim3(TPB), 0, st,
-      rho, rhot, blockoff, hist, blockhist, b2, jtab);
+      rho, rhot, blockoff, hist, blockhist, b2, jtab, mtw);
   return hipGetLastError();
 }
 
@@ -555,8 +572,8 @@ hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
 __global__ void __launch_bounds__(TPB)
 k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockofs,
     FastDiv fNg, double Lbox, real gs, size_t N, MrgState seed,
-    const uint32_t *__restrict__ jtab, real *__restrict__ x,
-    real *__restrict__ y, real *__restrict__ z, size_t ntot) {
+    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
+    real *__restrict__ x, real *__restrict__ y, real *__restrict__ z, size_t ntot) {
   __shared__ int16_t tile[PADIDX(CPB)];
   __shared__ uint32_t toff[TPB];
   __shared__ uint64_t ws[TPB / 64];
@@ -578,7 +595,7 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
   if (tot == 0) return;                 // uniform over the block
   toff[tid] = (uint32_t) excl;          // tot <= CPB * 1024
   const uint64_t tb = blockofs[blockIdx.x];
-  if (tid == 0) {
+  if (tid == 0 && !mtw) {
     MrgState g = seed;
     mrg_jump(g, 3 * tb, jtab);
     base = g;
@@ -600,18 +617,23 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
   int m = tile[PADIDX(c)];
   while (acc + (uint32_t) m <= l) { acc += (uint32_t) m; c++; m = tile[PADIDX(c)]; }
   int n = (int) (l - acc);
-  MrgState g = base;
-  mrg_jump(g, 3 * (uint64_t) l, jtab);
+  Draws ds;
+  ds.w = mtw;
+  ds.d = 3 * (tb + l);                  // MT19937 words start at draw 5N
+  if (!mtw) {
+    ds.g = base;
+    mrg_jump(ds.g, 3 * (uint64_t) l, jtab);
+  }
   uint32_t k, j;
   uint32_t ij = fastdiv_qr((uint32_t) (b0 + c), fNg, k);
   uint32_t i = fastdiv_qr(ij, fNg, j);
 #pragma unroll 1
   for (;;) {
-    double r = mrg_next(g) - 0.5;
+    double r = ds.next() - 0.5;
     real xx = (real) (((int) i + r) * (double) gs);
-    r = mrg_next(g) - 0.5;
+    r = ds.next() - 0.5;
     real yy = (real) (((int) j + r) * (double) gs);
-    r = mrg_next(g) - 0.5;
+    r = ds.next() - 0.5;
     real zz = (real) (((int) k + r) * (double) gs);
     if (xx < 0) xx = (real) ((double) xx + Lbox);
     if (yy < 0) yy = (real) ((double) yy + Lbox);
@@ -630,13 +652,13 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
 
 hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
-    real *x, real *y, real *z, size_t ntot, hipStream_t st) {
+    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st) {
   const size_t N = (size_t) Ng * Ng * Ng;
   if (N >= 0xffffffffull) return hipErrorInvalidValue;
   MrgState s5 = seed;
   mrg_jump(s5, (uint64_t) 5 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_positions, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      nt, blockofs, fastdiv_make((uint32_t) Ng), Lbox, gs, N, s5, jtab, x, y, z, ntot);
+      nt, blockofs, fastdiv_make((uint32_t) Ng), Lbox, gs, N, s5, jtab, mtw, x, y, z, ntot);
   return hipGetLastError();
 }
 
@@ -695,12 +717,13 @@ __global__ void __launch_bounds__(TPB)
 k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     const real *__restrict__ z, const real4 *__restrict__ vel, size_t ntot, int Ng, real igs, double vfac,
     double sigma, uint64_t base_draw, MrgState seed,
-    const uint32_t *__restrict__ jtab, real *__restrict__ vx,
+    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
+    real *__restrict__ vx,
     real *__restrict__ vy, real *__restrict__ vz) {
   __shared__ MrgState base;
   const size_t npair = (ntot + 1) / 2;
   const size_t q0blk = (size_t) xcd_item(blockIdx.x, gridDim.x) * TPB * PPT;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !mtw) {
     MrgState g = seed;
     mrg_jump(g, base_draw + 6 * q0blk, jtab);
     base = g;
@@ -708,8 +731,13 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
   __syncthreads();
   const size_t q0 = q0blk + (size_t) threadIdx.x * PPT;
   if (q0 >= npair) return;
-  MrgState g = base;
-  mrg_jump(g, 6 * (q0 - q0blk), jtab);
+  Draws g;
+  g.w = mtw;
+  g.d = 6 * q0;                         // MT19937 words start at draw 8N
+  if (!mtw) {
+    g.g = base;
+    mrg_jump(g.g, 6 * (q0 - q0blk), jtab);
+  }
 #pragma unroll 1
   for (int p = 0; p < PPT; p++) {
     const size_t q = q0 + p;
@@ -737,8 +765,8 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
 
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
     const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
-    MrgState seed, const uint32_t *jtab, real *vx, real *vy, real *vz,
-    hipStream_t st) {
+    MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
+    real *vz, hipStream_t st) {
   const size_t N = (size_t) Ng * Ng * Ng;
   const size_t npair = (ntot + 1) / 2;
   if (npair == 0) return hipSuccess;
@@ -746,7 +774,7 @@ hipError_t launch_velocities(const real *x, const real *y, const real *z,
   mrg_jump(s8, (uint64_t) 8 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_velocities, dim3(ez_blocks(npair, TPB * PPT)), dim3(TPB),
       0, st, x, y, z, vel, ntot, Ng, igs, vfac, sigma_v,
-      (uint64_t) 0, s8, jtab, vx, vy, vz);
+      (uint64_t) 0, s8, jtab, mtw, vx, vy, vz);
   return hipGetLastError();
 }
 

@@ -94,10 +94,10 @@ def test_pdf_walk_with_planted_ties(gpu_available, prec, seed):
 
 
 def run_pair(Ng, L, prec, seed=1, fixamp=False, iphase=False, logpk=False, params=None,
-             nexp=None, pk=None, nthread=1):
+             nexp=None, pk=None, nthread=1, rng=0):
     k, P = pk
-    g = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=seed, precision=prec)
-    o = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=seed, nthread=nthread, prec=prec,
+    g = EZmock(Lbox=L, Ngrid=Ng, rng=rng, seed=seed, precision=prec)
+    o = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=rng, seed=seed, nthread=nthread, prec=prec,
                         keep_spec=True)
     g.keep_spectrum(True)
     for e in (g, o):
@@ -116,14 +116,18 @@ def rel_rms(a, b):
     return np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2))
 
 
-@pytest.mark.parametrize("Ng,prec,fixamp,iphase,logpk", [
-    (32, "f64", False, False, False), (32, "f32", False, False, False),
-    (48, "f64", True, False, False), (48, "f32", False, True, False),
-    (64, "f64", False, False, True), (64, "f32", True, True, True),
+@pytest.mark.parametrize("Ng,prec,fixamp,iphase,logpk,rng", [
+    (32, "f64", False, False, False, 0), (32, "f32", False, False, False, 0),
+    (48, "f64", True, False, False, 0), (48, "f32", False, True, False, 0),
+    (64, "f64", False, False, True, 0), (64, "f32", True, True, True, 0),
+    (32, "f64", False, False, False, 1), (48, "f32", False, True, False, 1),
+    (64, "f64", True, False, True, 1),
 ])
-def test_pipeline_parity_small(gpu_available, pk_table, Ng, prec, fixamp, iphase, logpk):
+def test_pipeline_parity_small(gpu_available, pk_table, Ng, prec, fixamp, iphase, logpk, rng):
+    """Whole mock, GPU against the oracle, MRG32k3a (rng 0) and MT19937
+    (rng 1) streams."""
     g, o, cg, co = run_pair(Ng, 8.0 * Ng, prec, fixamp=fixamp, iphase=iphase, logpk=logpk,
-                            pk=pk_table)
+                            pk=pk_table, rng=rng, seed=1 if rng == 0 else 4357)
     sg, so = g.field("spec"), o.field("spec")
     if prec == "f32":
         # float spectra: the double->float rounding absorbs ulp-level device
@@ -216,11 +220,12 @@ def test_unsupported_configurations_fail_loudly(gpu_available, pk_table):
     g.setup_linear_pk(k, P)
     with pytest.raises(RuntimeError, match="invalid grid size"):
         g.create_dens_field_from_ic()
-    g = EZmock(Lbox=320., Ngrid=32, rng=1)
+    g = EZmock(Lbox=320., Ngrid=32, rng=0)
     g.eval_growth_params(0.5, 0.31)
     g.setup_linear_pk(k, P)
-    with pytest.raises(RuntimeError, match="failed to setup the random generator"):
-        g.create_dens_field_from_ic()
+    g.create_dens_field_from_ic()
+    with pytest.raises(RuntimeError):
+        g.populate_tracer(0.5, 10., 0.3, 150., 600, att_part=True)
 
 
 def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
