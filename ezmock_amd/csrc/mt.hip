@@ -3,9 +3,10 @@
 // A stream segment of words [first, first + n) is produced in three steps:
 //   1. the seed window is moved to word `first` with one jump (x^first mod
 //      phi, from mt_host.cpp);
-//   2. segment start windows every J words are built level by level: each
-//      workgroup of a level walks a chain of 16 jumps of the level's stride
-//      (16^k J), so S segments need 15 log16(S) jumps in sequence;
+//   2. segment start windows every J words are built level by level: at a
+//      level of stride 16^l J, window c of the coarser level gives windows
+//      16 c + k by one jump x^(k 16^l J) each, all in parallel, so S segments
+//      need log16(S) jumps in sequence;
 //   3. one workgroup per segment runs the twist recurrence in LDS and writes
 //      its J tempered words.
 // A jump applies g(A) = sum_i g_i A^i to a window with a blocked Horner
@@ -14,8 +15,10 @@
 // is a word-wise XOR over the set coefficients, and A^B acc is one parallel
 // generation of B words (B <= 227 keeps every new word a function of the
 // old window only).
+#include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "common.h"
@@ -41,74 +44,79 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
   return y;
 }
 
-// One workgroup per chain: out[c F + 0] = in[c], out[c F + k] = g(A) out[c F + k - 1];
-// chains stop at nout windows in total.
+// S <- g(A) S for the window S[0, 624) (S has room for 624 + B words),
+// with the coefficients gl in LDS; acc and nw are LDS scratch.
+__device__ void mt_jump_lds(const uint32_t *gl, uint32_t *S, uint32_t *acc,
+    uint32_t *nw) {
+  const int tid = threadIdx.x;
+  int qtop = -1;                         // highest block with a set coefficient
+  for (int i = MT_JUMP_WORDS - 1; i >= 0; i--)
+    if (gl[i]) { qtop = (i * 32) / MT_B; break; }
+  // S[624 .. 624 + B - 2] = next words of the input window
+  for (int j = tid; j < MT_B - 1; j += MT_TPB)
+    S[MT_N + j] = mt_f(S[j], S[j + 1], S[j + MT_M]);
+  for (int i = tid; i < MT_N; i += MT_TPB) acc[i] = 0;
+  __syncthreads();
+  for (int q = qtop; q >= 0; q--) {
+    // acc <- A^B acc
+    if (q != qtop) {
+      for (int j = tid; j < MT_B; j += MT_TPB) nw[j] = mt_f(acc[j], acc[j + 1], acc[j + MT_M]);
+      __syncthreads();
+      uint32_t keep[(MT_N + MT_TPB - 1) / MT_TPB];
+#pragma unroll
+      for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+        const int i = tid + u * MT_TPB;
+        if (i < MT_N) keep[u] = (i + MT_B < MT_N) ? acc[i + MT_B] : nw[i + MT_B - MT_N];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+        const int i = tid + u * MT_TPB;
+        if (i < MT_N) acc[i] = keep[u];
+      }
+    }
+    // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words)
+#pragma unroll
+    for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
+      const int i = tid + u * MT_TPB;
+      if (i >= MT_N) continue;
+      uint32_t t = 0;
+      for (int wq = 0; wq < MT_B / 32; wq++) {
+        const int gw = q * (MT_B / 32) + wq;
+        uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
+        while (bits) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1;
+          t ^= S[i + wq * 32 + b];
+        }
+      }
+      acc[i] ^= t;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < MT_N; i += MT_TPB) S[i] = (qtop >= 0) ? acc[i] : 0u;
+  __syncthreads();
+}
+
+// One workgroup per output window o = c fan + k (o < nout):
+// out[o] = g_k(A) in[c], g_k = polys[k] (k = 0: a copy of in[c]).
 __global__ void __launch_bounds__(MT_TPB)
-k_mt_chain(const uint32_t *__restrict__ g, const uint32_t *__restrict__ in,
+k_mt_fan(const uint32_t *__restrict__ polys, const uint32_t *__restrict__ in,
     int fan, size_t nout, uint32_t *__restrict__ out) {
   __shared__ uint32_t gl[MT_JUMP_WORDS];
   __shared__ uint32_t S[MT_N + MT_B];
   __shared__ uint32_t acc[MT_N], nw[MT_B];
   const int tid = threadIdx.x;
-  const size_t c = blockIdx.x;
-  for (int i = tid; i < MT_JUMP_WORDS; i += MT_TPB) gl[i] = g[i];
+  const size_t o = blockIdx.x;
+  if (o >= nout) return;                 // uniform over the workgroup
+  const size_t c = o / fan;
+  const int k = (int) (o % fan);
   for (int i = tid; i < MT_N; i += MT_TPB) S[i] = in[c * MT_N + i];
+  if (k > 0)
+    for (int i = tid; i < MT_JUMP_WORDS; i += MT_TPB) gl[i] = polys[(size_t) k * MT_JUMP_WORDS + i];
   __syncthreads();
-  int qtop = 0;                          // highest block with a set coefficient
-  for (int i = MT_JUMP_WORDS - 1; i >= 0; i--)
-    if (gl[i]) { qtop = (i * 32) / MT_B; break; }
-  for (int k = 0; k < fan; k++) {
-    const size_t o = c * fan + k;
-    if (o >= nout) break;
-    if (k > 0) {
-      // S[624 .. 624 + B - 2] = next words of the input window
-      for (int j = tid; j < MT_B - 1; j += MT_TPB)
-        S[MT_N + j] = mt_f(S[j], S[j + 1], S[j + MT_M]);
-      for (int i = tid; i < MT_N; i += MT_TPB) acc[i] = 0;
-      __syncthreads();
-      for (int q = qtop; q >= 0; q--) {
-        // acc <- A^B acc
-        if (q != qtop) {
-          for (int j = tid; j < MT_B; j += MT_TPB) nw[j] = mt_f(acc[j], acc[j + 1], acc[j + MT_M]);
-          __syncthreads();
-          uint32_t keep[(MT_N + MT_TPB - 1) / MT_TPB];
-#pragma unroll
-          for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-            const int i = tid + u * MT_TPB;
-            if (i < MT_N) keep[u] = (i + MT_B < MT_N) ? acc[i + MT_B] : nw[i + MT_B - MT_N];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-            const int i = tid + u * MT_TPB;
-            if (i < MT_N) acc[i] = keep[u];
-          }
-        }
-        // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words)
-#pragma unroll
-        for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-          const int i = tid + u * MT_TPB;
-          if (i >= MT_N) continue;
-          uint32_t t = 0;
-          for (int wq = 0; wq < MT_B / 32; wq++) {
-            const int gw = q * (MT_B / 32) + wq;
-            uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
-            while (bits) {
-              const int b = __ffs(bits) - 1;
-              bits &= bits - 1;
-              t ^= S[i + wq * 32 + b];
-            }
-          }
-          acc[i] ^= t;
-        }
-        __syncthreads();
-      }
-      for (int i = tid; i < MT_N; i += MT_TPB) S[i] = acc[i];
-      __syncthreads();
-    }
-    for (int i = tid; i < MT_N; i += MT_TPB) out[o * MT_N + i] = S[i];
-    __syncthreads();
-  }
+  if (k > 0) mt_jump_lds(gl, S, acc, nw);
+  for (int i = tid; i < MT_N; i += MT_TPB) out[o * MT_N + i] = S[i];
 }
 
 // One workgroup per segment s: words [s J, min((s + 1) J, n)) of the stream
@@ -144,27 +152,34 @@ k_mt_fill(const uint32_t *__restrict__ starts, uint64_t J, uint64_t n,
 
 namespace {
 std::mutex g_mu;
-std::map<std::pair<int, uint64_t>, uint32_t *> g_dev_poly;   // (device, e) -> g
+// (device, stride, fan) -> fan polynomials x^(k stride) mod phi, k < fan
+std::map<std::tuple<int, uint64_t, int>, uint32_t *> g_dev_polys;
 
-const uint32_t *device_poly(uint64_t e) {
+const uint32_t *device_fan_polys(uint64_t stride, int fan) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const auto key = std::make_tuple(dev, stride, fan);
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_dev_poly.find({dev, e});
-    if (it != g_dev_poly.end()) return it->second;
+    auto it = g_dev_polys.find(key);
+    if (it != g_dev_polys.end()) return it->second;
   }
-  const uint32_t *h = mt_jump_poly(e);
-  if (!h) return nullptr;
+  std::vector<uint32_t> h((size_t) fan * MT_JUMP_WORDS, 0);
+  for (int k = 1; k < fan; k++) {
+    const uint32_t *g = mt_jump_poly((uint64_t) k * stride);
+    if (!g) return nullptr;
+    std::copy(g, g + MT_JUMP_WORDS, h.begin() + (size_t) k * MT_JUMP_WORDS);
+  }
   uint32_t *d = nullptr;
-  if (hipMalloc(&d, MT_JUMP_WORDS * sizeof(uint32_t)) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, h, MT_JUMP_WORDS * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMalloc(&d, h.size() * sizeof(uint32_t)) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
     (void) hipFree(d);
     return nullptr;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  g_dev_poly[{dev, e}] = d;
-  return d;
+  auto res = g_dev_polys.emplace(key, d);
+  if (!res.second) (void) hipFree(d);    // another thread stored it first
+  return res.first->second;
 }
 }  // namespace
 
@@ -190,27 +205,26 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
   auto fail = [&](hipError_t er) { (void) hipFreeAsync(wa, st); (void) hipFreeAsync(wb, st); return er; };
   if ((e = hipMemcpyAsync(wa, w0, sizeof(w0), hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);  // w0 is on the host stack
-  if (first) {
-    const uint32_t *g = device_poly(first);
+  if (first) {                           // window at word `first`
+    const uint32_t *g = device_fan_polys(first, 2);
     if (!g) return fail(hipErrorInvalidValue);
-    hipLaunchKernelGGL(k_mt_chain, dim3(1), dim3(MT_TPB), 0, st, g, wa, 2, (size_t) 2, wb);
+    hipLaunchKernelGGL(k_mt_fan, dim3(2), dim3(MT_TPB), 0, st, g, wa, 2, (size_t) 2, wb);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(wa, wb + MT_N, MT_N * sizeof(uint32_t), hipMemcpyDeviceToDevice, st)) != hipSuccess)
       return fail(e);
   }
-  // level l: count windows at stride J * FAN^(L-1-l) from count/FAN at the coarser one
-  uint64_t have = 1;
+  // level l: windows at stride J FAN^(L-1-l), each one jump x^(k stride)
+  // (k < FAN) from a window of the coarser level
   for (int l = 0; l < L; l++) {
     uint64_t stride = J;
     for (int k = 0; k < L - 1 - l; k++) stride *= MT_FAN;
     const uint64_t need = (nseg + stride / J - 1) / (stride / J);
-    const uint32_t *g = device_poly(stride);
+    const uint32_t *g = device_fan_polys(stride, MT_FAN);
     if (!g) return fail(hipErrorInvalidValue);
-    hipLaunchKernelGGL(k_mt_chain, dim3((unsigned) have), dim3(MT_TPB), 0, st, g, wa,
+    hipLaunchKernelGGL(k_mt_fan, dim3((unsigned) need), dim3(MT_TPB), 0, st, g, wa,
         MT_FAN, (size_t) need, wb);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     uint32_t *t = wa; wa = wb; wb = t;
-    have = need;
   }
   hipLaunchKernelGGL(k_mt_fill, dim3((unsigned) nseg), dim3(MT_TPB), 0, st, wa, J, n, out);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
