@@ -190,12 +190,16 @@ def run_serving(nreq=400, rate=20.0, nslot=4, budget=200e9, seed=7):
     pool.close()
     lat = np.array([r["latency"] for r in res]) * 1e3
     ok = [r for r in res if not r["err"]]
+    # the slowest requests, split into queue wait and compute time
+    slow = [{"Ng": int(grids[i]), "wait_ms": round((res[i]["t_start"] - res[i]["t_submit"]) * 1e3, 1),
+             "run_ms": round((res[i]["t_end"] - res[i]["t_start"]) * 1e3, 1)}
+            for i in np.argsort(-lat)[:5]]
     return {"requests": nreq, "rate_per_s": rate, "slots": nslot,
             "errors": len(res) - len(ok), "seconds": round(dt, 2),
             "mocks_per_s": round(len(ok) / dt, 2),
             "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
             "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
-            "latency_ms_max": round(float(lat.max()), 2)}
+            "latency_ms_max": round(float(lat.max()), 2), "slowest": slow}
 
 
 def cpu_baseline(precision):
@@ -280,6 +284,18 @@ def main():
         "kernel_ms_per_mock": {n: round(t / args.steps, 3) for n, (t, c) in kt.items()},
     }
     if not args.no_extra:
+        # the pool runs before the large single mocks, so that the serving
+        # latencies do not follow the release of a 150 GB instance
+        try:
+            extra["batch_64x256"] = run_batch()
+            log(f"bench: batch {extra['batch_64x256']}")
+        except Exception as e:
+            extra["batch_64x256"] = f"failed: {e}"
+        try:
+            extra["serving"] = run_serving(nreq=args.serve_requests)
+            log(f"bench: serving {extra['serving']}")
+        except Exception as e:
+            extra["serving"] = f"failed: {e}"
         try:
             d2, n2, _, _ = run_gpu(1024, 2500., 42, params, density, "f32", 2, 1, profile=False)
             extra["ms_per_mock_1024_f32"] = round(d2 * 1e3 / 2, 2)
@@ -297,16 +313,6 @@ def main():
             log(f"bench: 1536^3 {extra['large_1536_f32']}")
         except Exception as e:
             extra["large_1536_f32"] = f"failed: {e}"
-        try:
-            extra["batch_64x256"] = run_batch()
-            log(f"bench: batch {extra['batch_64x256']}")
-        except Exception as e:
-            extra["batch_64x256"] = f"failed: {e}"
-        try:
-            extra["serving"] = run_serving(nreq=args.serve_requests)
-            log(f"bench: serving {extra['serving']}")
-        except Exception as e:
-            extra["serving"] = f"failed: {e}"
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle sample ...")
