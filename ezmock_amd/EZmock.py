@@ -289,6 +289,12 @@ class EZmock:
     def profile(self, on=True):
         self.set_option(2, on)
 
+    def cic_method(self, method):
+        """0 or 1: particle binning (default); 2: tile scan of the displaced
+        lattice wherever the displacement range admits it, else binning."""
+        if self.set_option(3, int(method)):
+            raise ValueError(f"unknown CIC method {method}")
+
     def kernel_times(self):
         n = self._L.EZmock_ext_kernel_times(self._ez, None, None, 0, None, 0)
         ms = np.zeros(max(n, 1))

@@ -59,6 +59,11 @@ struct Mesh {
   uint32_t *cic_counts = nullptr, *cic_list = nullptr;
   uint64_t *cic_off = nullptr, *cic_cur = nullptr, *cic_scan = nullptr;
   size_t cic_list_cap = 0;
+  // CIC tile scan: bounds blocks, extrema keys, the scan status (pinned)
+  real4 *cic_bnd = nullptr;
+  rkey *cic_gkey = nullptr, *cic_status_h = nullptr;
+  int cic_path = 0;             // 0, 1: particle binning; 2: tile scan where valid
+  struct { bool on; int Ng; real igs; real *rho; bool mom; } cic_pend = {false, 0, 0, nullptr, false};
   double *ptab = nullptr;
   long ptab_version = -1;
   cplx *twN = nullptr, *twh = nullptr;
@@ -166,6 +171,8 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->hist); dfree(&m->counter); dfree(&m->blockhist);
   dfree(&m->cic_counts); dfree(&m->cic_list); m->cic_list_cap = 0; dfree(&m->cic_off);
   dfree(&m->cic_cur); dfree(&m->cic_scan);
+  dfree(&m->cic_bnd); dfree(&m->cic_gkey);
+  if (m->cic_status_h) { (void) hipHostFree(m->cic_status_h); m->cic_status_h = nullptr; }
   free_tracer_scratch(m);
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
@@ -242,18 +249,14 @@ int ensure_twiddles(Mesh *m, int Ng) {
 
 // CIC painting of the displaced particles (cic.hip): density into rho
 // (unless null) and, when with_mom, the three momentum fields.
-int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
+// CIC by particle binning: count, scan, scatter, tiles.
+int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   const CicGeom g = cic_geometry(Ng, igs);
-  if (g.N >= 0xffffffffull) {
-    fprintf(stderr, "EZmock: grids of 2^32 cells or more are not supported\n");
-    return EZMOCK_ERR_ARG_NGRID;
-  }
   const size_t nk = cic_nkeys(g);
   int e;
   if ((e = dalloc(&m->cic_counts, nk)) || (e = dalloc(&m->cic_off, nk + 1)) ||
       (e = dalloc(&m->cic_cur, nk)) || (e = dalloc(&m->cic_scan, scan_tmp_words(nk) + 8)))
     return e;
-  if (with_mom && (e = dalloc(&m->vel4, g.N))) return e;
   {
     Scope s(m, "cic_bin_count");
     KCHK(launch_cic_count(m->psi[0], m->psi[1], m->psi[2], g, m->cic_counts,
@@ -281,6 +284,63 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
         m->cic_list, rho, m->rho, with_mom ? m->vel4 : nullptr, m->st));
   }
   return 0;
+}
+
+// Density (into rho, unless null) and, with with_mom, the normalised
+// momentum fields of the displacement fields psi (dens_field.c:233-310,
+// pop_tracer.c:593-682).  By particle binning, or (option) by tile scan
+// where the displacement range allows it, decided on the device: the
+// painting is then complete only after cic_finish(), called after the next
+// synchronisation of the stream, which repeats it by binning when the scan
+// declined.
+int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
+  const CicGeom g = cic_geometry(Ng, igs);
+  if (g.N >= 0xffffffffull) {
+    fprintf(stderr, "EZmock: grids of 2^32 cells or more are not supported\n");
+    return EZMOCK_ERR_ARG_NGRID;
+  }
+  int e;
+  if (with_mom && (e = dalloc(&m->vel4, g.N))) return e;
+  if (m->cic_path != 2) return paint_cic_binned(m, Ng, igs, rho, with_mom);
+  if ((e = dalloc(&m->cic_bnd, 2 * cic_nblocks(g))) ||
+      (e = dalloc(&m->cic_gkey, CIC_GKEY_WORDS)))
+    return e;
+  if (!m->cic_status_h &&
+      hipHostMalloc((void **) &m->cic_status_h, sizeof(rkey)) != hipSuccess) {
+    m->cic_status_h = nullptr;
+    return EZMOCK_ERR_MEMORY;
+  }
+  {
+    Scope s(m, "cic_bounds");
+    KCHK(launch_cic_bounds(m->psi[0], m->psi[1], m->psi[2], g, m->cic_bnd, m->cic_gkey,
+        m->st));
+  }
+  {
+    Scope s(m, with_mom ? (rho ? "cic_scan_rho_mom" : "cic_scan_mom") : "cic_scan_rho");
+    KCHK(launch_cic_scan(m->psi[0], m->psi[1], m->psi[2], g, m->cic_bnd, m->cic_gkey,
+        rho, m->rho, with_mom ? m->vel4 : nullptr, m->st));
+  }
+  HIP_OK(hipMemcpyAsync(m->cic_status_h, m->cic_gkey + CIC_GKEY_STATUS, sizeof(rkey),
+      hipMemcpyDeviceToHost, m->st));
+  m->cic_pend.on = true;
+  m->cic_pend.Ng = Ng;
+  m->cic_pend.igs = igs;
+  m->cic_pend.rho = rho;
+  m->cic_pend.mom = with_mom;
+  return 0;
+}
+
+// After a stream synchronisation that follows paint_cic: paint by binning
+// if the tile scan declined.
+int cic_finish(Mesh *m) {
+  if (!m->cic_pend.on) return 0;
+  m->cic_pend.on = false;
+  if (!*m->cic_status_h) return 0;
+  int e;
+  if ((e = paint_cic_binned(m, m->cic_pend.Ng, m->cic_pend.igs, m->cic_pend.rho,
+                            m->cic_pend.mom)))
+    return e;
+  return sync_and_collect(m);
 }
 
 // The (j, rows) passes of the c2r of a displacement component whose first
@@ -543,7 +603,9 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if (!m->mom_valid) {
     // fields given by reference: the momentum fields follow the caller's
     // current arrays, the density stays the one of create_dens_field
-    if ((e = paint_cic(m, Ng, igs, nullptr, true))) return e;
+    if ((e = paint_cic(m, Ng, igs, nullptr, true)) || (e = sync_and_collect(m)) ||
+        (e = cic_finish(m)))
+      return e;
     m->mom_valid = !m->psi_ref;
   }
   const uint32_t *mtw_vel = nullptr;
@@ -909,7 +971,7 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
   if ((e = paint_cic(m, Ng, (real) (conf->Ng / conf->Lbox), m->rho, !m->psi_ref)))
     return (*err = e);
   m->mom_valid = !m->psi_ref;
-  if ((e = sync_and_collect(m))) return (*err = e);
+  if ((e = sync_and_collect(m)) || (e = cic_finish(m))) return (*err = e);
   m->fixamp = fixamp;
   m->iphase = iphase;
   m->have_field = true;
@@ -1212,6 +1274,7 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
   Mesh *m = MESH(ez);
   if (option == 1) m->keep_spec = value != 0;
   else if (option == 2) m->profile = value != 0;
+  else if (option == 3 && value >= 0 && value <= 2) m->cic_path = (int) value;
   else return EZMOCK_ERR_ARG_PARAMS;
   return EZMOCK_SUCCESS;
 }

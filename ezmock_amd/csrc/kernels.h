@@ -35,9 +35,28 @@ struct CicGeom {
   int ntx, nty, ntz;            // tiles per dimension
   size_t N;                     // particles (< 2^32)
   FastDiv fNg, fntz;            // division by Ng and by ntz
+  // bounds blocks of the Lagrangian lattice (tile scan): BX x BY x BZ
+  // lattice points, powers of two
+  int BX, BY, BZ, lbx, lby, lbz;
+  int nbx, nby, nbz;
 };
 CicGeom cic_geometry(int Ng, real igs);
 size_t cic_nkeys(const CicGeom &g);
+size_t cic_nblocks(const CicGeom &g);
+// Words of the scan key array: displacement bounds (3 minima, 3 maxima as
+// order-preserving keys), a non-finite flag and the scan status.
+#define CIC_GKEY_WORDS 8
+#define CIC_GKEY_STATUS 7
+// Per bounds block, the minimum and maximum of psi_a * igs over its lattice
+// points (bnd: 2 cic_nblocks(g) real4), and their extrema over the grid.
+hipError_t launch_cic_bounds(const real *p0, const real *p1, const real *p2,
+    const CicGeom &g, real4 *bnd, rkey *gkey, hipStream_t st);
+// Tile scan: the density (and optional momentum) tiles from the lattice
+// points whose bounds admit them.  When the displacement range makes the
+// scan inapplicable, nothing is written and gkey[CIC_GKEY_STATUS] is set.
+hipError_t launch_cic_scan(const real *p0, const real *p1, const real *p2,
+    const CicGeom &g, const real4 *bnd, rkey *gkey, real *rho,
+    const real *rho_norm, real4 *vel, hipStream_t st);
 hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, uint32_t *counts, uint64_t *off, uint64_t *cursor,
     uint64_t *scan_tmp, hipStream_t st);

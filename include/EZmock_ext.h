@@ -54,7 +54,11 @@ int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err);
 int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
 
 /* Options: 1 = keep a copy of the generated spectrum (tests);
-            2 = record per-kernel device times with HIP events. */
+            2 = record per-kernel device times with HIP events;
+            3 = CIC method: 0 or 1 = particle binning (default), 2 = tile
+                scan of the displaced lattice wherever the displacement
+                range admits it (no particle lists), else binning.  Both
+                give the same fields. */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
