@@ -241,7 +241,7 @@ k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
   const size_t base = (size_t) o * outer_stride + kk0;
   constexpr int E = L * TK;
   if (MODE == 0) {
-#pragma unroll 4
+#pragma unroll 16
     for (int e = threadIdx.x; e < E; e += NT) {
       const int n = e / TK, c = e % TK;
       lds[n * TK + c] = buf[base + (size_t) n * stride + c];
@@ -251,7 +251,7 @@ k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
     // rhok has buf's layout: plain, batched loads times n_a; the kz = 0 /
     // Nyquist slot (kk = 0, first column tile only) is redone afterwards
     const int ko = (o <= Ngh) ? o : o - Ng;
-#pragma unroll 4
+#pragma unroll 16
     for (int e = threadIdx.x; e < E; e += NT) {
       const int n = e / TK, c = e % TK;
       const int kn = (n <= Ngh) ? n : n - Ng;
@@ -384,7 +384,7 @@ k_fft_rows_c2r(C *__restrict__ buf, const C *__restrict__ twN,
   const size_t row0 = (size_t) blockIdx.x * TR;
   C *g = buf + row0 * Lh;
   constexpr int E = Lh * TR;
-#pragma unroll 4
+#pragma unroll 16
   for (int e = threadIdx.x; e < E; e += NT) lds[e] = g[e];
   __syncthreads();
   // Hermitian pre-processing, pairs (m, Lh - m) for m in [0, Lh/2].
@@ -447,7 +447,7 @@ k_fft_rows_r2c(C *__restrict__ buf, const C *__restrict__ twN,
   const size_t row0 = (size_t) blockIdx.x * TR;
   C *g = buf + row0 * Lh;
   constexpr int E = Lh * TR;
-#pragma unroll 4
+#pragma unroll 16
   for (int e = threadIdx.x; e < E; e += NT) lds[e] = g[e];
   __syncthreads();
   lds_fft<C, Lh, TR, NT, -1, 1, Lh, false>(lds, twh);

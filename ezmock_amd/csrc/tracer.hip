@@ -57,7 +57,8 @@ k_scan_chunks(const uint32_t *__restrict__ in, uint64_t *__restrict__ out,
   uint64_t s = 0;
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
-    v[q] = (base + q < n) ? in[base + q] : 0u;
+    const uint32_t t = in[base + q < n ? base + q : n - 1];   // no branch per load
+    v[q] = (base + q < n) ? t : 0u;
     s += v[q];
   }
   uint64_t tot;
@@ -78,7 +79,8 @@ k_scan_top(uint64_t *__restrict__ s, size_t n, uint64_t *__restrict__ total) {
   uint64_t v[CPT], t = 0;
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
-    v[q] = (base + q < n) ? s[base + q] : 0;
+    const uint64_t x = s[base + q < n ? base + q : n - 1];
+    v[q] = (base + q < n) ? x : 0;
     t += v[q];
   }
   uint64_t tot;
@@ -118,11 +120,19 @@ k_count_valid(const real *__restrict__ rho, real thres, size_t N,
     uint32_t *__restrict__ blockcnt) {
   __shared__ uint32_t ws[TPB / 64];
   const size_t base = (size_t) blockIdx.x * CPB;
-  uint32_t c = 0;
-#pragma unroll 4
+  // unconditional loads at clamped indices: all in flight together (a
+  // guarded load waits for its data at the branch join)
+  real v[CPT];
+#pragma unroll
   for (int q = 0; q < CPT; q++) {
     const size_t i = base + (size_t) q * TPB + threadIdx.x;
-    if (i < N && rho[i] >= thres) c++;
+    v[q] = rho[i < N ? i : N - 1];
+  }
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = base + (size_t) q * TPB + threadIdx.x;
+    if (i < N && v[q] >= thres) c++;
   }
   uint32_t tot;
   (void) block_excl_scan<uint32_t>(c, tot, ws);
@@ -182,11 +192,12 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
   const size_t b0 = (size_t) blockIdx.x * CPB;
   const int tid = threadIdx.x;
   if (tid < EZ_NUM_LOGB_BIN) lh[tid] = 0;
-#pragma unroll 4
+#pragma unroll
   for (int q = 0; q < CPT; q++) {
     const int c = q * TPB + tid;
     const size_t i = b0 + c;
-    tile[PADIDX(c)] = (i < bp.N) ? rho[i] : (real) 0;
+    const real t = rho[i < bp.N ? i : bp.N - 1];
+    tile[PADIDX(c)] = (i < bp.N) ? t : (real) 0;
   }
   __syncthreads();
   // thread tid owns cells c = 16 tid .. 16 tid + 15 of the block: rank the
@@ -297,7 +308,8 @@ k_compact(const real *__restrict__ rhot, real thres, size_t N,
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
     const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
-    v[q] = (i < N) ? rhot[i] : (real) 0;
+    const real t = rhot[i < N ? i : N - 1];
+    v[q] = (i < N) ? t : (real) 0;
     if (i < N && v[q] >= thres) c++;
   }
   uint32_t tot;
@@ -355,7 +367,8 @@ k_compact_at(const real *__restrict__ rhot, real thres, size_t N,
 #pragma unroll
   for (int q = 0; q < CPT; q++) {
     const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
-    v[q] = (i < N) ? rhot[i] : (real) 0;
+    const real t = rhot[i < N ? i : N - 1];
+    v[q] = (i < N) ? t : (real) 0;
     if (i < N && v[q] >= thres) c++;
   }
   uint32_t tot;
@@ -393,10 +406,16 @@ k_sort_hist(const rkey *__restrict__ in, size_t n, int shift,
   h[threadIdx.x] = 0;
   __syncthreads();
   const size_t base = (size_t) blockIdx.x * CPB;
-#pragma unroll 4
+  rkey kv[SORT_ITEMS];
+#pragma unroll
   for (int it = 0; it < SORT_ITEMS; it++) {
     const size_t e = base + (size_t) it * TPB + threadIdx.x;
-    if (e < n) atomicAdd(&h[(uint32_t) (in[e] >> shift) & 255u], 1u);
+    kv[it] = in[e < n ? e : n - 1];               // all loads in flight
+  }
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const size_t e = base + (size_t) it * TPB + threadIdx.x;
+    if (e < n) atomicAdd(&h[(uint32_t) (kv[it] >> shift) & 255u], 1u);
   }
   __syncthreads();
   gh[(size_t) threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
@@ -421,7 +440,8 @@ k_sort_scatter(const rkey *__restrict__ in, rkey *__restrict__ out, size_t n,
   for (int it = 0; it < SORT_ITEMS; it++) {
     const size_t e = base + (size_t) it * TPB + tid;
     const bool valid = e < n;
-    const rkey key = valid ? in[e] : (rkey) 0;
+    const rkey kv = in[valid ? e : n - 1];
+    const rkey key = valid ? kv : (rkey) 0;
     const uint32_t d = (uint32_t) (key >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -532,11 +552,17 @@ k_assign_nt(const real *__restrict__ rhot, size_t N, const rkey *__restrict__ vb
   const rkey vlast = sb[nmax - 1];
   const size_t base = (size_t) blockIdx.x * CPB;
   uint32_t sum = 0;
+  real vv[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = base + (size_t) q * TPB + threadIdx.x;
+    vv[q] = rhot[i < N ? i : N - 1];              // all loads in flight
+  }
 #pragma unroll 4
   for (int q = 0; q < CPT; q++) {
     const size_t i = base + (size_t) q * TPB + threadIdx.x;
     if (i >= N) break;
-    const real v = rhot[i];
+    const real v = vv[q];
     int16_t c = 0;
     if (v > 0) {
       const rkey u = real_key(v);
@@ -580,11 +606,12 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
   __shared__ MrgState base;
   const size_t b0 = (size_t) blockIdx.x * CPB;
   const int tid = threadIdx.x;
-#pragma unroll 4
+#pragma unroll
   for (int q = 0; q < CPT; q++) {
     const int c = q * TPB + tid;
     const size_t i = b0 + c;
-    tile[PADIDX(c)] = (i < N) ? nt[i] : (int16_t) 0;
+    const int16_t t = nt[i < N ? i : N - 1];
+    tile[PADIDX(c)] = (i < N) ? t : (int16_t) 0;
   }
   __syncthreads();
   uint64_t cnt = 0;
