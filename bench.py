@@ -54,6 +54,10 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
         "cic_tiles_rho_mom": 8 * F,            # read psi x3, write rho + {rho, u x3}
         "cic_tiles_rho": 4 * F,
         "cic_tiles_mom": 8 * F,                # read psi x3 + rho, write {rho, u x3}
+        "cic_bounds": 3 * F,                   # tile-scan option: read psi x3
+        "cic_scan_rho_mom": 8 * F,
+        "cic_scan_rho": 4 * F,
+        "cic_scan_mom": 8 * F,
         "velocities": (32 + 6) * s * ntr,      # 8 corners x 4 fields + r/w
     }
     return table.get(kernel)
