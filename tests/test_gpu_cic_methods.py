@@ -135,4 +135,4 @@ def test_non_finite_displacement_takes_binning(gpu_available, pk_table):
 def test_cic_method_option_is_validated(gpu_available):
     g = EZmock(Lbox=100., Ngrid=16, rng=0, seed=1, precision="f32")
     with pytest.raises(ValueError):
-        g.cic_method(3)
+        g.cic_method(4)
