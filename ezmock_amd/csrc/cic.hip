@@ -108,8 +108,10 @@ __device__ __forceinline__ void span_tile(const TileSpan &s, const CicGeom &g,
   if (uz == g.ntz) uz = 0;
 }
 
+// (tile counts and window extents are below 2^24: full-rate 24-bit products)
 __device__ __forceinline__ uint32_t tile_key(const CicGeom &g, int ux, int uy, int uz) {
-  return (uint32_t) ((ux * g.nty + uy) * g.ntz + uz);
+  return __umul24(__umul24((uint32_t) ux, (uint32_t) g.nty) + (uint32_t) uy,
+                  (uint32_t) g.ntz) + (uint32_t) uz;
 }
 
 // Local tile window of a workgroup of CIC_PPB consecutive particles: the
@@ -155,7 +157,8 @@ __device__ __forceinline__ int window_slot(const Window &w, const CicGeom &g,
   if (dx < 0) dx += g.ntx;
   if (dy < 0) dy += g.nty;
   if (dx >= w.wx || dy >= w.wy) return -1;
-  return (dx * w.wy + dy) * g.ntz + uz;
+  return (int) (__umul24(__umul24((uint32_t) dx, (uint32_t) w.wy) + (uint32_t) dy,
+                         (uint32_t) g.ntz) + (uint32_t) uz);
 }
 
 __device__ __forceinline__ uint32_t slot_key(const Window &w, const CicGeom &g,
@@ -199,11 +202,13 @@ __device__ __forceinline__ uint32_t wave_count_add(uint32_t *wcnt, bool active,
   return 0;
 }
 
-// The particle's tile span (n = 0 for indices past N).
+// The particle's tile span (n = 0 for indices past N); (i, j, k) its
+// lattice coordinates.
 __device__ __forceinline__ int particle_span(const real *q0, const real *q1,
-    const real *q2, int u, uint32_t p, const CicGeom &g, TileSpan &s) {
+    const real *q2, int u, uint32_t p, uint32_t i, uint32_t j, uint32_t k,
+    const CicGeom &g, TileSpan &s) {
   if ((size_t) p >= g.N) return 0;
-  const ParticleCell c = cell_from(q0[u], q1[u], q2[u], p, g);
+  const ParticleCell c = cell_at(q0[u], q1[u], q2[u], i, j, k, g);
   s = tile_span(c, g);
   return s.ex * s.ey * s.ez;
 }
@@ -233,12 +238,32 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
     q[1][u] = p1[p];
     q[2][u] = p2[p];
   }
+  // lattice coordinates of the thread's particles: one division for the
+  // first, then steps of CIC_TPB = (di Ng + dj) Ng + dk with carries
+  uint32_t li[CIC_PPT], lj[CIC_PPT], lk[CIC_PPT];
+  {
+    uint32_t k, j;
+    const uint32_t p = b0 + threadIdx.x;
+    const uint32_t ij = fastdiv_qr(p < g.N ? p : 0u, g.fNg, k);
+    uint32_t i = fastdiv_qr(ij, g.fNg, j);
+#pragma unroll
+    for (int u = 0; u < CIC_PPT; u++) {
+      li[u] = i; lj[u] = j; lk[u] = k;
+      k += (uint32_t) g.sdk;
+      const uint32_t ck = k >= (uint32_t) g.Ng;
+      k -= ck ? (uint32_t) g.Ng : 0u;
+      j += (uint32_t) g.sdj + ck;
+      const uint32_t cj = j >= (uint32_t) g.Ng;
+      j -= cj ? (uint32_t) g.Ng : 0u;
+      i += (uint32_t) g.sdi + cj;
+    }
+  }
   // pass 1: count the window's tiles
 #pragma unroll
   for (int u = 0; u < CIC_PPT; u++) {
     const uint32_t p = b0 + (uint32_t) (u * CIC_TPB) + threadIdx.x;
     TileSpan sp;
-    const int n = particle_span(q[0], q[1], q[2], u, p, g, sp);
+    const int n = particle_span(q[0], q[1], q[2], u, p, li[u], lj[u], lk[u], g, sp);
     for (int v = 0; __ballot(v < n); v++) {
       int s = -1;
       if (v < n) {
@@ -267,7 +292,7 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
   for (int u = 0; u < CIC_PPT; u++) {
     const uint32_t p = b0 + (uint32_t) (u * CIC_TPB) + threadIdx.x;
     TileSpan sp;
-    const int n = particle_span(q[0], q[1], q[2], u, p, g, sp);
+    const int n = particle_span(q[0], q[1], q[2], u, p, li[u], lj[u], lk[u], g, sp);
     for (int v = 0; __ballot(v < n); v++) {
       int s = -1;
       int ux = 0, uy = 0, uz = 0;
@@ -828,6 +853,10 @@ CicGeom cic_geometry(int Ng, real igs) {
   g.N = (size_t) Ng * Ng * Ng;
   g.fNg = fastdiv_make((uint32_t) Ng);
   g.fntz = fastdiv_make((uint32_t) g.ntz);
+  // CIC_TPB as a lattice step (Ng >= 8 > CIC_TPB^(1/3))
+  g.sdk = CIC_TPB % Ng;
+  g.sdj = (CIC_TPB / Ng) % Ng;
+  g.sdi = CIC_TPB / (Ng * Ng);
   g.BX = tile_dim(Ng, 4);
   g.BY = tile_dim(Ng, 4);
   g.BZ = tile_dim(Ng, 8);
