@@ -696,13 +696,21 @@ hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
 // Gaussian scatter of pair q from draws 8N + 6q.
 #define PPT 4
 
-__device__ __forceinline__ void gather_vel(const real4 *__restrict__ vel,
-    int Ng, real igs, double vfac, real px, real py, real pz, real *v) {
+// CIC corners of a tracer and their {rho, u} vectors: loads issued here,
+// used by corners_vel (so that the loads of several tracers are in flight
+// together).
+struct Corners {
+  real4 f[8];
+  real wx0, wx1, wy0, wy1, wz0, wz1;
+};
+
+__device__ __forceinline__ void corners_load(const real4 *__restrict__ vel,
+    int Ng, real igs, real px, real py, real pz, Corners &k) {
   const real xx = px * igs, yy = py * igs, zz = pz * igs;
   int x0 = (int) xx, y0 = (int) yy, z0 = (int) zz;
-  const real wx1 = xx - (real) x0, wx0 = 1 - wx1;
-  const real wy1 = yy - (real) y0, wy0 = 1 - wy1;
-  const real wz1 = zz - (real) z0, wz0 = 1 - wz1;
+  k.wx1 = xx - (real) x0; k.wx0 = 1 - k.wx1;
+  k.wy1 = yy - (real) y0; k.wy0 = 1 - k.wy1;
+  k.wz1 = zz - (real) z0; k.wz0 = 1 - k.wz1;
   if (x0 == Ng) x0 = 0;
   if (y0 == Ng) y0 = 0;
   if (z0 == Ng) z0 = 0;
@@ -718,9 +726,13 @@ __device__ __forceinline__ void gather_vel(const real4 *__restrict__ vel,
     ((size_t) x1 * Ng + y0) * Ng + z0, ((size_t) x1 * Ng + y0) * Ng + z1,
     ((size_t) x1 * Ng + y1) * Ng + z0, ((size_t) x1 * Ng + y1) * Ng + z1};
   // per corner {rho, u0, u1, u2}: u already normalised by rho (disp_cic)
-  real4 f[8];
 #pragma unroll
-  for (int q = 0; q < 8; q++) f[q] = vel[c[q]];
+  for (int q = 0; q < 8; q++) k.f[q] = vel[c[q]];
+}
+
+__device__ __forceinline__ void corners_vel(const Corners &k, double vfac, real *v) {
+  const real4 *f = k.f;
+  const real wx0 = k.wx0, wx1 = k.wx1, wy0 = k.wy0, wy1 = k.wy1, wz0 = k.wz0, wz1 = k.wz1;
   real r[8];
   // r_q = rho * wx * wy * wz, evaluated left to right as assign_vel does
   r[0] = f[0].x * wx0 * wy0 * wz0; r[1] = f[1].x * wx0 * wy0 * wz1;
@@ -770,8 +782,14 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     const size_t q = q0 + p;
     if (q >= npair) break;
     const size_t a = 2 * q, b = a + 1;
+    const bool hb = b < ntot;             // else: the last, unpaired tracer
+    const size_t bb = hb ? b : a;
+    // both tracers' corner loads first, then the draws and sums
+    Corners ka, kb;
+    corners_load(vel, Ng, igs, x[a], y[a], z[a], ka);
+    corners_load(vel, Ng, igs, x[bb], y[bb], z[bb], kb);
     real va[3], vb3[3];
-    gather_vel(vel, Ng, igs, vfac, x[a], y[a], z[a], va);
+    corners_vel(ka, vfac, va);
     double g1, g2;
     gauss_pair(g, sigma, g1, g2);
     va[0] = (real) ((double) va[0] + g1);
@@ -779,8 +797,8 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     gauss_pair(g, sigma, g1, g2);
     va[2] = (real) ((double) va[2] + g1);
     vx[a] = va[0]; vy[a] = va[1]; vz[a] = va[2];
-    if (b < ntot) {                       // else: the last, unpaired tracer
-      gather_vel(vel, Ng, igs, vfac, x[b], y[b], z[b], vb3);
+    if (hb) {
+      corners_vel(kb, vfac, vb3);
       vb3[0] = (real) ((double) vb3[0] + g2);
       gauss_pair(g, sigma, g1, g2);
       vb3[1] = (real) ((double) vb3[1] + g1);
