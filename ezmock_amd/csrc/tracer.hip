@@ -436,12 +436,17 @@ k_sort_scatter(const rkey *__restrict__ in, rkey *__restrict__ out, size_t n,
   __syncthreads();
   const size_t base = (size_t) blockIdx.x * CPB;
   const uint64_t lt = (1ull << lane) - 1;
-#pragma unroll 1
+  rkey kv[SORT_ITEMS];                    // all key loads in flight first
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; it++) {
+    const size_t e = base + (size_t) it * TPB + tid;
+    kv[it] = in[e < n ? e : n - 1];
+  }
+#pragma unroll
   for (int it = 0; it < SORT_ITEMS; it++) {
     const size_t e = base + (size_t) it * TPB + tid;
     const bool valid = e < n;
-    const rkey kv = in[valid ? e : n - 1];
-    const rkey key = valid ? kv : (rkey) 0;
+    const rkey key = valid ? kv[it] : (rkey) 0;
     const uint32_t d = (uint32_t) (key >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
