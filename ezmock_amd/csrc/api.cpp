@@ -45,6 +45,11 @@ struct Mesh {
   real *psi_host[3] = {nullptr, nullptr, nullptr};
   bool dev_ready = false;
   hipStream_t st = nullptr;
+  // two more streams (default priority) for the per-component FFT passes,
+  // forked from and joined into st with events
+  hipStream_t sx[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  bool fft_streams = true;
   const uint32_t *jtab = nullptr;
   // fields (device)
   real *psi[3] = {nullptr, nullptr, nullptr};
@@ -123,6 +128,11 @@ int device_ready(Mesh *m) {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  for (int i = 0; i < 2; i++) {
+    HIP_OK(hipStreamCreateWithFlags(&m->sx[i], hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&m->ev_join[i], hipEventDisableTiming));
+  }
+  HIP_OK(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
   {
     std::lock_guard<std::mutex> lk(g_jtab_mu);
     auto it = g_jtab.find(dev);
@@ -184,21 +194,30 @@ void free_mesh_device(Mesh *m) {
   m->pending.clear();
   if (m->st) (void) hipStreamDestroy(m->st);
   m->st = nullptr;
+  for (int i = 0; i < 2; i++) {
+    if (m->sx[i]) (void) hipStreamDestroy(m->sx[i]);
+    if (m->ev_join[i]) (void) hipEventDestroy(m->ev_join[i]);
+    m->sx[i] = nullptr;
+    m->ev_join[i] = nullptr;
+  }
+  if (m->ev_fork) (void) hipEventDestroy(m->ev_fork);
+  m->ev_fork = nullptr;
 }
 
 // Kernel-class timing with HIP events on the instance stream (option 2).
 struct Scope {
-  Mesh *m; Timer t; bool on;
-  Scope(Mesh *mm, const char *name) : m(mm), on(mm->profile) {
+  Mesh *m; Timer t; bool on; hipStream_t s;
+  Scope(Mesh *mm, const char *name, hipStream_t ss = nullptr)
+      : m(mm), on(mm->profile), s(ss ? ss : mm->st) {
     if (!on) return;
     t.name = name;
     (void) hipEventCreate(&t.a);
     (void) hipEventCreate(&t.b);
-    (void) hipEventRecord(t.a, m->st);
+    (void) hipEventRecord(t.a, s);
   }
   ~Scope() {
     if (!on) return;
-    (void) hipEventRecord(t.b, m->st);
+    (void) hipEventRecord(t.b, s);
     m->pending.push_back(t);
   }
 };
@@ -345,15 +364,15 @@ int cic_finish(Mesh *m) {
 
 // The (j, rows) passes of the c2r of a displacement component whose first
 // pass (axis i, fft_first_pass3 from the generated spectrum) is done.
-int c2r_finish(Mesh *m, int Ng, real *out) {
+int c2r_finish(Mesh *m, int Ng, real *out, hipStream_t st) {
   cplx *buf = (cplx *) out;
   {
-    Scope s(m, "fft_pass_j");
-    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, m->st));
+    Scope s(m, "fft_pass_j", st);
+    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, st));
   }
   {
-    Scope s(m, "fft_rows_c2r");
-    KCHK(fft_row_pass(buf, m->twN, m->twh, Ng, +1, m->st));
+    Scope s(m, "fft_rows_c2r", st);
+    KCHK(fft_row_pass(buf, m->twN, m->twh, Ng, +1, st));
   }
   return 0;
 }
@@ -956,10 +975,27 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
         return (*err = EZMOCK_ERR_UNKNOWN);
       }
     }
+    // the three components' remaining passes are independent: components
+    // 1 and 2 on their own streams, forked after the first pass and joined
+    // before the CIC
+    hipStream_t cs[3] = {m->st, m->st, m->st};
+    if (m->fft_streams) {
+      HIP_OK(hipEventRecord(m->ev_fork, m->st));
+      for (int i = 0; i < 2; i++) {
+        HIP_OK(hipStreamWaitEvent(m->sx[i], m->ev_fork, 0));
+        cs[1 + i] = m->sx[i];
+      }
+    }
     for (int c = 0; c < 3; c++) {
-      if ((e = c2r_finish(m, Ng, m->psi[c]))) { m->ny = ny_save; return (*err = e); }
+      if ((e = c2r_finish(m, Ng, m->psi[c], cs[c]))) { m->ny = ny_save; return (*err = e); }
     }
     m->ny = ny_save;
+    if (m->fft_streams) {
+      for (int i = 0; i < 2; i++) {
+        HIP_OK(hipEventRecord(m->ev_join[i], m->sx[i]));
+        HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[i], 0));
+      }
+    }
     if (delta) {
       const real norm = 1 / sqrt((double) N);
       Scope s(m, "scale");
@@ -1275,6 +1311,7 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
   if (option == 1) m->keep_spec = value != 0;
   else if (option == 2) m->profile = value != 0;
   else if (option == 3 && value >= 0 && value <= 2) m->cic_path = (int) value;
+  else if (option == 4) m->fft_streams = value != 0;
   else return EZMOCK_ERR_ARG_PARAMS;
   return EZMOCK_SUCCESS;
 }

@@ -58,7 +58,9 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
             3 = CIC method: 0 or 1 = particle binning (default), 2 = tile
                 scan of the displaced lattice wherever the displacement
                 range admits it (no particle lists), else binning.  Both
-                give the same fields. */
+                give the same fields;
+            4 = run the last two FFT passes of the three displacement
+                components on three streams (1, default) or on one (0). */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
