@@ -978,8 +978,11 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     // the three components' remaining passes are independent: components
     // 1 and 2 on their own streams, forked after the first pass and joined
     // before the CIC
+    // (one stream while per-kernel timers are on, so that their intervals
+    // do not overlap)
+    const bool fork = m->fft_streams && !m->profile;
     hipStream_t cs[3] = {m->st, m->st, m->st};
-    if (m->fft_streams) {
+    if (fork) {
       HIP_OK(hipEventRecord(m->ev_fork, m->st));
       for (int i = 0; i < 2; i++) {
         HIP_OK(hipStreamWaitEvent(m->sx[i], m->ev_fork, 0));
@@ -990,7 +993,7 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
       if ((e = c2r_finish(m, Ng, m->psi[c], cs[c]))) { m->ny = ny_save; return (*err = e); }
     }
     m->ny = ny_save;
-    if (m->fft_streams) {
+    if (fork) {
       for (int i = 0; i < 2; i++) {
         HIP_OK(hipEventRecord(m->ev_join[i], m->sx[i]));
         HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[i], 0));

@@ -60,7 +60,8 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
                 range admits it (no particle lists), else binning.  Both
                 give the same fields;
             4 = run the last two FFT passes of the three displacement
-                components on three streams (1, default) or on one (0). */
+                components on three streams (1, default) or on one (0);
+                one stream while option 2 is on. */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
