@@ -246,3 +246,19 @@ def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
             assert np.array_equal(a, b)
         for a, b in zip(cg[3:], co[3:]):
             assert np.abs(a - b).max() < 1e-6
+
+
+def test_fft_component_streams_same_fields(gpu_available, pk_table):
+    """The three displacement components' last FFT passes on three streams
+    (default) or on one stream (option 4 = 0): identical displacements."""
+    k, P = pk_table
+    out = {}
+    for fs in (1, 0):
+        g = EZmock(Lbox=384., Ngrid=64, rng=0, seed=5, precision="f32")
+        g.set_option(4, fs)
+        g.eval_growth_params(0.5, 0.31)
+        g.setup_linear_pk(k, P)
+        g.create_dens_field_from_ic()
+        out[fs] = [g.field(f) for f in ("psi_x", "psi_y", "psi_z")]
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(a, b)
