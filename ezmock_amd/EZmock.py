@@ -290,8 +290,10 @@ class EZmock:
         self.set_option(2, on)
 
     def cic_method(self, method):
-        """0 or 1: particle binning (default); 2: tile scan of the displaced
-        lattice wherever the displacement range admits it, else binning."""
+        """0: particle binning into fixed list ranges (default; exact ranges
+        after an overflow); 1: particle binning with exact ranges; 2: tile
+        scan of the displaced lattice wherever the displacement range admits
+        it, else binning."""
         if self.set_option(3, int(method)):
             raise ValueError(f"unknown CIC method {method}")
 

@@ -67,8 +67,17 @@ struct Mesh {
   // CIC tile scan: bounds blocks, extrema keys, the scan status (pinned)
   real4 *cic_bnd = nullptr;
   rkey *cic_gkey = nullptr, *cic_status_h = nullptr;
-  int cic_path = 0;             // 0, 1: particle binning; 2: tile scan where valid
-  struct { bool on; int Ng; real igs; real *rho; bool mom; } cic_pend = {false, 0, 0, nullptr, false};
+  // binning with fixed list ranges per tile: overflow flag (device, pinned
+  // host copy); cleared for the instance after the first overflow
+  uint32_t *cic_ovf = nullptr, *cic_ovf_h = nullptr;
+  bool cic_fixed_ok = true;
+  // 0: binning, fixed ranges (exact after an overflow); 1: binning, exact
+  // ranges; 2: tile scan where valid
+  int cic_path = 0;
+  // painting completed only after the next synchronisation: kind 1 = tile
+  // scan (redo by binning if declined), 2 = fixed ranges (redo with exact
+  // ranges on overflow)
+  struct { int kind; int Ng; real igs; real *rho; bool mom; } cic_pend = {0, 0, 0, nullptr, false};
   double *ptab = nullptr;
   long ptab_version = -1;
   cplx *twN = nullptr, *twh = nullptr;
@@ -181,8 +190,9 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->hist); dfree(&m->counter); dfree(&m->blockhist);
   dfree(&m->cic_counts); dfree(&m->cic_list); m->cic_list_cap = 0; dfree(&m->cic_off);
   dfree(&m->cic_cur); dfree(&m->cic_scan);
-  dfree(&m->cic_bnd); dfree(&m->cic_gkey);
+  dfree(&m->cic_bnd); dfree(&m->cic_gkey); dfree(&m->cic_ovf);
   if (m->cic_status_h) { (void) hipHostFree(m->cic_status_h); m->cic_status_h = nullptr; }
+  if (m->cic_ovf_h) { (void) hipHostFree(m->cic_ovf_h); m->cic_ovf_h = nullptr; }
   free_tracer_scratch(m);
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
@@ -268,7 +278,22 @@ int ensure_twiddles(Mesh *m, int Ng) {
 
 // CIC painting of the displaced particles (cic.hip): density into rho
 // (unless null) and, when with_mom, the three momentum fields.
-// CIC by particle binning: count, scan, scatter, tiles.
+void cic_pending(Mesh *m, int kind, int Ng, real igs, real *rho, bool with_mom) {
+  m->cic_pend.kind = kind;
+  m->cic_pend.Ng = Ng;
+  m->cic_pend.igs = igs;
+  m->cic_pend.rho = rho;
+  m->cic_pend.mom = with_mom;
+}
+
+// List entries per tile with fixed ranges: 4 per cell of the tile, about
+// three times the mean (1.35 per cell); dense tiles rarely exceed twice it
+// at cell sizes of a few Mpc/h.
+#define CIC_FIXED_PER_CELL 4
+
+// CIC by particle binning: scatter into fixed ranges per tile and tiles,
+// checked for overflow after the next synchronisation (no count pass, no
+// host synchronisation); or count, scan, scatter, tiles with exact ranges.
 int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   const CicGeom g = cic_geometry(Ng, igs);
   const size_t nk = cic_nkeys(g);
@@ -276,6 +301,36 @@ int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   if ((e = dalloc(&m->cic_counts, nk)) || (e = dalloc(&m->cic_off, nk + 1)) ||
       (e = dalloc(&m->cic_cur, nk)) || (e = dalloc(&m->cic_scan, scan_tmp_words(nk) + 8)))
     return e;
+  const uint32_t fcap = (uint32_t) (CIC_FIXED_PER_CELL * g.TX * g.TY * g.TZ);
+  if (m->cic_path == 0 && m->cic_fixed_ok && nk < (1ull << 27)) {
+    const size_t need = nk * fcap;
+    if (m->cic_list_cap < need) {
+      dfree(&m->cic_list);
+      if ((e = dalloc(&m->cic_list, need))) { m->cic_list_cap = 0; return e; }
+      m->cic_list_cap = need;
+    }
+    if ((e = dalloc(&m->cic_ovf, 1))) return e;
+    if (!m->cic_ovf_h &&
+        hipHostMalloc((void **) &m->cic_ovf_h, sizeof(uint32_t)) != hipSuccess) {
+      m->cic_ovf_h = nullptr;
+      return EZMOCK_ERR_MEMORY;
+    }
+    {
+      Scope s(m, "cic_bin_scatter");
+      KCHK(launch_cic_fixed_init(m->cic_cur, g, fcap, m->cic_ovf, m->st));
+      KCHK(launch_cic_scatter(m->psi[0], m->psi[1], m->psi[2], g, m->cic_cur,
+          m->cic_list, fcap, m->cic_ovf, m->st));
+    }
+    {
+      Scope s(m, with_mom ? (rho ? "cic_tiles_rho_mom" : "cic_tiles_mom") : "cic_tiles_rho");
+      KCHK(launch_cic_tiles(m->psi[0], m->psi[1], m->psi[2], g, m->cic_cur,
+          m->cic_list, rho, m->rho, with_mom ? m->vel4 : nullptr, fcap, m->st));
+    }
+    HIP_OK(hipMemcpyAsync(m->cic_ovf_h, m->cic_ovf, sizeof(uint32_t),
+        hipMemcpyDeviceToHost, m->st));
+    cic_pending(m, 2, Ng, igs, rho, with_mom);
+    return 0;
+  }
   {
     Scope s(m, "cic_bin_count");
     KCHK(launch_cic_count(m->psi[0], m->psi[1], m->psi[2], g, m->cic_counts,
@@ -295,12 +350,12 @@ int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   {
     Scope s(m, "cic_bin_scatter");
     KCHK(launch_cic_scatter(m->psi[0], m->psi[1], m->psi[2], g, m->cic_cur,
-        m->cic_list, m->st));
+        m->cic_list, 0u, nullptr, m->st));
   }
   {
     Scope s(m, with_mom ? (rho ? "cic_tiles_rho_mom" : "cic_tiles_mom") : "cic_tiles_rho");
     KCHK(launch_cic_tiles(m->psi[0], m->psi[1], m->psi[2], g, m->cic_off,
-        m->cic_list, rho, m->rho, with_mom ? m->vel4 : nullptr, m->st));
+        m->cic_list, rho, m->rho, with_mom ? m->vel4 : nullptr, 0u, m->st));
   }
   return 0;
 }
@@ -308,10 +363,9 @@ int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
 // Density (into rho, unless null) and, with with_mom, the normalised
 // momentum fields of the displacement fields psi (dens_field.c:233-310,
 // pop_tracer.c:593-682).  By particle binning, or (option) by tile scan
-// where the displacement range allows it, decided on the device: the
-// painting is then complete only after cic_finish(), called after the next
-// synchronisation of the stream, which repeats it by binning when the scan
-// declined.
+// where the displacement range allows it.  The painting is complete only
+// after cic_finish(), called after the next synchronisation of the stream,
+// which repeats it when the scan declined or a fixed list range overflowed.
 int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   const CicGeom g = cic_geometry(Ng, igs);
   if (g.N >= 0xffffffffull) {
@@ -320,6 +374,7 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   }
   int e;
   if (with_mom && (e = dalloc(&m->vel4, g.N))) return e;
+  m->cic_pend.kind = 0;
   if (m->cic_path != 2) return paint_cic_binned(m, Ng, igs, rho, with_mom);
   if ((e = dalloc(&m->cic_bnd, 2 * cic_nblocks(g))) ||
       (e = dalloc(&m->cic_gkey, CIC_GKEY_WORDS)))
@@ -341,25 +396,29 @@ int paint_cic(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   }
   HIP_OK(hipMemcpyAsync(m->cic_status_h, m->cic_gkey + CIC_GKEY_STATUS, sizeof(rkey),
       hipMemcpyDeviceToHost, m->st));
-  m->cic_pend.on = true;
-  m->cic_pend.Ng = Ng;
-  m->cic_pend.igs = igs;
-  m->cic_pend.rho = rho;
-  m->cic_pend.mom = with_mom;
+  cic_pending(m, 1, Ng, igs, rho, with_mom);
   return 0;
 }
 
-// After a stream synchronisation that follows paint_cic: paint by binning
-// if the tile scan declined.
+// After a stream synchronisation that follows paint_cic: paint again when
+// the tile scan declined (by binning) or a fixed list range overflowed
+// (with exact ranges, and so on for this instance).
 int cic_finish(Mesh *m) {
-  if (!m->cic_pend.on) return 0;
-  m->cic_pend.on = false;
-  if (!*m->cic_status_h) return 0;
-  int e;
-  if ((e = paint_cic_binned(m, m->cic_pend.Ng, m->cic_pend.igs, m->cic_pend.rho,
-                            m->cic_pend.mom)))
-    return e;
-  return sync_and_collect(m);
+  while (m->cic_pend.kind) {
+    const int kind = m->cic_pend.kind;
+    m->cic_pend.kind = 0;
+    if (kind == 1 && !*m->cic_status_h) return 0;
+    if (kind == 2) {
+      if (!*m->cic_ovf_h) return 0;
+      m->cic_fixed_ok = false;
+    }
+    int e;
+    if ((e = paint_cic_binned(m, m->cic_pend.Ng, m->cic_pend.igs, m->cic_pend.rho,
+                              m->cic_pend.mom)) ||
+        (e = sync_and_collect(m)))
+      return e;
+  }
+  return 0;
 }
 
 // The (j, rows) passes of the c2r of a displacement component whose first
@@ -1313,7 +1372,10 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
   Mesh *m = MESH(ez);
   if (option == 1) m->keep_spec = value != 0;
   else if (option == 2) m->profile = value != 0;
-  else if (option == 3 && value >= 0 && value <= 2) m->cic_path = (int) value;
+  else if (option == 3 && value >= 0 && value <= 2) {
+    m->cic_path = (int) value;
+    m->cic_fixed_ok = true;
+  }
   else if (option == 4) m->fft_streams = value != 0;
   else return EZMOCK_ERR_ARG_PARAMS;
   return EZMOCK_SUCCESS;

@@ -213,11 +213,15 @@ __device__ __forceinline__ int particle_span(const real *q0, const real *q1,
   return s.ex * s.ey * s.ez;
 }
 
+// SCATTER with cap > 0: tile t owns list[t cap, (t + 1) cap) (cursor
+// starts at t cap); entries past the end are dropped and *ovf set.  In that
+// mode wbase holds (tile key << 36) | position.
 template <int SCATTER>
 __global__ void __launch_bounds__(CIC_TPB)
 k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
     const real *__restrict__ p2, CicGeom g, uint32_t *__restrict__ counts,
-    unsigned long long *__restrict__ cursor, uint32_t *__restrict__ list) {
+    unsigned long long *__restrict__ cursor, uint32_t *__restrict__ list, uint32_t cap,
+    uint32_t *__restrict__ ovf) {
   __shared__ uint32_t wcnt[WMAX];
   __shared__ unsigned long long wbase[SCATTER ? WMAX : 1];
   const uint32_t b0 = blockIdx.x * CIC_PPB;
@@ -283,8 +287,11 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
   }
   // scatter: reserve each window tile's range, then place the indices
   for (int s = threadIdx.x; s < nslot; s += CIC_TPB) {
-    if (wcnt[s])
-      wbase[s] = atomicAdd(&cursor[slot_key(w, g, s)], (unsigned long long) wcnt[s]);
+    if (wcnt[s]) {
+      const uint32_t key = slot_key(w, g, s);
+      const unsigned long long b = atomicAdd(&cursor[key], (unsigned long long) wcnt[s]);
+      wbase[s] = cap ? (((unsigned long long) key << 36) | b) : b;
+    }
     wcnt[s] = 0;
   }
   __syncthreads();
@@ -302,10 +309,18 @@ k_cic_bin(const real *__restrict__ p0, const real *__restrict__ p1,
       }
       const uint32_t r = wave_count_add<true>(wcnt, s >= 0, s);
       if (v < n) {
-        unsigned long long pos;
-        if (s >= 0) pos = wbase[s] + r;
-        else pos = atomicAdd(&cursor[tile_key(g, ux, uy, uz)], 1ull);
-        list[pos] = p;
+        unsigned long long pos, key;
+        if (s >= 0) {
+          const unsigned long long b = wbase[s];
+          key = b >> 36;
+          pos = (cap ? (b & ((1ull << 36) - 1)) : b) + r;
+        }
+        else {
+          key = tile_key(g, ux, uy, uz);
+          pos = atomicAdd(&cursor[key], 1ull);
+        }
+        if (!cap || pos < (key + 1) * cap) list[pos] = p;
+        else *ovf = 1u;
       }
     }
   }
@@ -409,7 +424,7 @@ __global__ void __launch_bounds__(CIC_TILE_TPB)
 k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
     const real *__restrict__ p2, CicGeom g, const uint64_t *__restrict__ off,
     const uint32_t *__restrict__ list, real *__restrict__ rho,
-    const real *__restrict__ rho_norm, real4 *__restrict__ vel, int tpw) {
+    const real *__restrict__ rho_norm, real4 *__restrict__ vel, int tpw, uint32_t cap) {
   constexpr int TC = TXX * TYY * TZZ;
   constexpr int NF = MOM ? 4 : 1;
   // double accumulators: LDS double atomics are several times faster than
@@ -426,7 +441,17 @@ k_cic_tile(const real *__restrict__ p0, const real *__restrict__ p1,
   __syncthreads();
   const int tz = t % g.ntz, ty = (t / g.ntz) % g.nty, tx = t / (g.ntz * g.nty);
   const int ox = tx * TXX, oy = ty * TYY, oz = tz * TZZ;
-  const uint64_t s0 = off[t], e0 = off[t + 1];
+  // list range: [off[t], off[t + 1]), or with fixed ranges [t cap,
+  // min(cursor[t], (t + 1) cap)) where off is the cursor array
+  uint64_t s0, e0;
+  if (cap) {
+    s0 = (uint64_t) t * cap;
+    e0 = off[t] < s0 + cap ? off[t] : s0 + cap;
+  }
+  else {
+    s0 = off[t];
+    e0 = off[t + 1];
+  }
   constexpr int U = 4;
   for (uint64_t qb = s0; qb < e0; qb += (uint64_t) U * CIC_TILE_TPB) {
     size_t pv[U];
@@ -871,25 +896,25 @@ size_t cic_nblocks(const CicGeom &g) { return (size_t) g.nbx * g.nby * g.nbz; }
 template <int MOM, int A, int B, int C>
 static void launch_tile(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    const real *rho_norm, real4 *vel, hipStream_t st) {
+    const real *rho_norm, real4 *vel, uint32_t cap, hipStream_t st) {
   const unsigned nt = (unsigned) (g.ntx * g.nty * g.ntz);
   const int tpw = CIC_TILES_PER_WG;
   hipLaunchKernelGGL((k_cic_tile<MOM, A, B, C>), dim3((nt + tpw - 1) / tpw),
-      dim3(CIC_TILE_TPB), 0, st, p0, p1, p2, g, off, list, rho, rho_norm, vel, tpw);
+      dim3(CIC_TILE_TPB), 0, st, p0, p1, p2, g, off, list, rho, rho_norm, vel, tpw, cap);
 }
 
 template <int MOM>
 static hipError_t dispatch_tile(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    const real *rho_norm, real4 *vel, hipStream_t st) {
+    const real *rho_norm, real4 *vel, uint32_t cap, hipStream_t st) {
   if (g.TX == 8 && g.TY == 8 && g.TZ == 32)
-    launch_tile<MOM, 8, 8, 32>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+    launch_tile<MOM, 8, 8, 32>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
   else if (g.TX == 8 && g.TY == 8 && g.TZ == 16)
-    launch_tile<MOM, 8, 8, 16>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+    launch_tile<MOM, 8, 8, 16>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
   else if (g.TX == 8 && g.TY == 8 && g.TZ == 8)
-    launch_tile<MOM, 8, 8, 8>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+    launch_tile<MOM, 8, 8, 8>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
   else if (g.TX == 4 && g.TY == 4 && g.TZ == 4)
-    launch_tile<MOM, 4, 4, 4>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+    launch_tile<MOM, 4, 4, 4>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -907,7 +932,7 @@ hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
   if ((e = launch_zero(counts, nk * sizeof(uint32_t), st)) != hipSuccess) return e;
   const unsigned nb = (unsigned) ((g.N + CIC_PPB - 1) / CIC_PPB);
   hipLaunchKernelGGL(k_cic_bin<0>, dim3(nb), dim3(CIC_TPB), 0, st, p0, p1, p2, g,
-      counts, (unsigned long long *) nullptr, (uint32_t *) nullptr);
+      counts, (unsigned long long *) nullptr, (uint32_t *) nullptr, 0u, (uint32_t *) nullptr);
   if ((e = launch_scan_u32(counts, off, nk, scan_tmp, st)) != hipSuccess) return e;
   // off[nk] = total list length (end of the last list); cursor = off
   const size_t nch = (nk + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
@@ -919,10 +944,27 @@ hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
 // Particle indices into the lists of the tiles they touch (list: off[nkeys]
 // u32, between N and 8 N; about 1.35 N for 8 x 8 x 16 tiles).
 hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
-    const CicGeom &g, uint64_t *cursor, uint32_t *list, hipStream_t st) {
+    const CicGeom &g, uint64_t *cursor, uint32_t *list, uint32_t cap, uint32_t *ovf,
+    hipStream_t st) {
+  if (cap && (!ovf || cic_nkeys(g) >= (1ull << 27))) return hipErrorInvalidValue;
   const unsigned nb = (unsigned) ((g.N + CIC_PPB - 1) / CIC_PPB);
   hipLaunchKernelGGL(k_cic_bin<1>, dim3(nb), dim3(CIC_TPB), 0, st, p0, p1, p2, g,
-      (uint32_t *) nullptr, (unsigned long long *) cursor, list);
+      (uint32_t *) nullptr, (unsigned long long *) cursor, list, cap, ovf);
+  return hipGetLastError();
+}
+
+__global__ void k_cic_fixed_init(uint64_t *__restrict__ cursor, size_t nk,
+    uint32_t cap, uint32_t *__restrict__ ovf) {
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nk) cursor[i] = (uint64_t) i * cap;
+  if (i == 0) *ovf = 0u;
+}
+
+hipError_t launch_cic_fixed_init(uint64_t *cursor, const CicGeom &g, uint32_t cap,
+    uint32_t *ovf, hipStream_t st) {
+  const size_t nk = cic_nkeys(g);
+  hipLaunchKernelGGL(k_cic_fixed_init, dim3(ez_blocks(nk, 256)), dim3(256), 0, st,
+      cursor, nk, cap, ovf);
   return hipGetLastError();
 }
 
@@ -930,10 +972,10 @@ hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
 // null, the normalised momentum fields.
 hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    const real *rho_norm, real4 *vel, hipStream_t st) {
+    const real *rho_norm, real4 *vel, uint32_t cap, hipStream_t st) {
   if (vel && !rho && !rho_norm) return hipErrorInvalidValue;
-  if (vel) return dispatch_tile<1>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
-  return dispatch_tile<0>(p0, p1, p2, g, off, list, rho, rho_norm, vel, st);
+  if (vel) return dispatch_tile<1>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
+  return dispatch_tile<0>(p0, p1, p2, g, off, list, rho, rho_norm, vel, cap, st);
 }
 
 hipError_t launch_cic_bounds(const real *p0, const real *p1, const real *p2,

@@ -61,14 +61,23 @@ hipError_t launch_cic_scan(const real *p0, const real *p1, const real *p2,
 hipError_t launch_cic_count(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, uint32_t *counts, uint64_t *off, uint64_t *cursor,
     uint64_t *scan_tmp, hipStream_t st);
+// cap = 0: exact list ranges from the count pass (cursor = off).  cap > 0:
+// tile t owns list[t cap, (t + 1) cap) after launch_cic_fixed_init; entries
+// past a tile's range are dropped and *ovf set (the painting must then be
+// redone with exact ranges).
 hipError_t launch_cic_scatter(const real *p0, const real *p1, const real *p2,
-    const CicGeom &g, uint64_t *cursor, uint32_t *list, hipStream_t st);
+    const CicGeom &g, uint64_t *cursor, uint32_t *list, uint32_t cap, uint32_t *ovf,
+    hipStream_t st);
+hipError_t launch_cic_fixed_init(uint64_t *cursor, const CicGeom &g, uint32_t cap,
+    uint32_t *ovf, hipStream_t st);
 // vel (optional): per cell {rho_c, u0, u1, u2}, u_a = m_a / rho_c where
 // rho_c > REAL_TOL and m_a otherwise (m_a the painted momentum); rho_c is
 // the density painted in the same pass, or rho_norm when rho is null.
+// off: exact ranges [off[t], off[t + 1]) (cap = 0), or the cursor array of
+// fixed ranges [t cap, min(cursor[t], (t + 1) cap)) (cap > 0).
 hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,
     const CicGeom &g, const uint64_t *off, const uint32_t *list, real *rho,
-    const real *rho_norm, real4 *vel, hipStream_t st);
+    const real *rho_norm, real4 *vel, uint32_t cap, hipStream_t st);
 
 // tracer.hip
 struct BiasParams {

@@ -55,10 +55,12 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
 
 /* Options: 1 = keep a copy of the generated spectrum (tests);
             2 = record per-kernel device times with HIP events;
-            3 = CIC method: 0 or 1 = particle binning (default), 2 = tile
-                scan of the displaced lattice wherever the displacement
-                range admits it (no particle lists), else binning.  Both
-                give the same fields;
+            3 = CIC method: 0 = particle binning into fixed list ranges
+                per tile (default; exact ranges for the instance after an
+                overflow), 1 = particle binning with exact ranges (count
+                pass), 2 = tile scan of the displaced lattice wherever the
+                displacement range admits it (no particle lists), else
+                binning.  All give the same fields;
             4 = run the last two FFT passes of the three displacement
                 components on three streams (1, default) or on one (0);
                 one stream while option 2 is on. */

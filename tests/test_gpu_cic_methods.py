@@ -1,5 +1,6 @@
-"""CIC painting by particle binning (default) and by tile scan
-(EZmock_ext_set_option 3 = 2): both against the oracle, and against each other, on generated fields,
+"""CIC painting by particle binning into fixed list ranges (default), by
+binning with exact ranges (EZmock_ext_set_option 3 = 1) and by tile scan
+(option 3 = 2): all against the oracle, and against each other, on generated fields,
 on random displacement fields of large range (forced scan, and the ranges
 the scan declines), and on non-finite input (the scan declines; binning
 paints).  Tolerances: FP64 density within 1e-10 relative of the oracle;
@@ -58,7 +59,7 @@ def test_methods_against_oracle(gpu_available, pk_table, Ng, L, z, prec):
     o, co = oracle(Ng, L, z, prec, pk_table)
     ro = o.field("rho")
     res = {}
-    for method in (0, 2):
+    for method in (0, 1, 2):
         g, names, cg = run_method(method, Ng, L, z, prec, pk_table)
         assert used_scan(names) == (method == 2)
         rg = g.field("rho")
@@ -68,7 +69,7 @@ def test_methods_against_oracle(gpu_available, pk_table, Ng, L, z, prec):
             d = rg.astype(np.float64) - ro
             assert np.sqrt(np.mean(d ** 2) / np.mean(ro.astype(np.float64) ** 2)) < 1e-5
         res[method] = (rg, g.field("nt"), cg)
-    for method in (2,):
+    for method in (1, 2):
         rg, nt, cg = res[method]
         rb, ntb, cb = res[0]
         # the same sums in another order: FP64 to rounding, FP32 (double
@@ -136,3 +137,27 @@ def test_cic_method_option_is_validated(gpu_available):
     g = EZmock(Lbox=100., Ngrid=16, rng=0, seed=1, precision="f32")
     with pytest.raises(ValueError):
         g.cic_method(4)
+
+
+
+def test_fixed_lists_overflow_falls_back(gpu_available, pk_table):
+    """Displacements that pull every particle into a few cells overflow the
+    fixed list ranges of their tiles: the painting is redone with exact
+    ranges (a count pass appears) and matches the oracle."""
+    Ng = 32
+    L = float(Ng)
+    q = np.arange(Ng, dtype=np.float64)
+    i, j, k = np.meshgrid(q, q, q, indexing="ij")
+    disp = [(-0.9 * (a - Ng / 2)).ravel() for a in (i, j, k)]
+    o = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=3, nthread=4, prec="f64")
+    o.eval_growth_params(0.5, 0.31)
+    o.create_dens_field_from_disp(*disp)
+    g = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=3, precision="f64")
+    g.profile(True)
+    g.eval_growth_params(0.5, 0.31)
+    g.create_dens_field_from_disp(*disp)
+    names = set(g.kernel_times())
+    assert "cic_bin_count" in names
+    ro = o.field("rho")
+    assert np.abs(g.field("rho") - ro).max() <= 1e-10 * np.abs(ro).max()
+    o.close()
