@@ -71,6 +71,7 @@ struct Mesh {
   // host copy); cleared for the instance after the first overflow
   uint32_t *cic_ovf = nullptr, *cic_ovf_h = nullptr;
   bool cic_fixed_ok = true;
+  uint32_t cic_fcap = 0;        // entries per tile range (0: the default)
   // 0: binning, fixed ranges (exact after an overflow); 1: binning, exact
   // ranges; 2: tile scan where valid
   int cic_path = 0;
@@ -288,8 +289,10 @@ void cic_pending(Mesh *m, int kind, int Ng, real igs, real *rho, bool with_mom) 
 
 // List entries per tile with fixed ranges: 4 per cell of the tile, about
 // three times the mean (1.35 per cell); dense tiles rarely exceed twice it
-// at cell sizes of a few Mpc/h.
+// at cell sizes of a few Mpc/h.  After an overflow the ranges are resized
+// from the exact counts, up to CIC_FIXED_MAX_PER_CELL.
 #define CIC_FIXED_PER_CELL 4
+#define CIC_FIXED_MAX_PER_CELL 6
 
 // CIC by particle binning: scatter into fixed ranges per tile and tiles,
 // checked for overflow after the next synchronisation (no count pass, no
@@ -301,7 +304,8 @@ int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
   if ((e = dalloc(&m->cic_counts, nk)) || (e = dalloc(&m->cic_off, nk + 1)) ||
       (e = dalloc(&m->cic_cur, nk)) || (e = dalloc(&m->cic_scan, scan_tmp_words(nk) + 8)))
     return e;
-  const uint32_t fcap = (uint32_t) (CIC_FIXED_PER_CELL * g.TX * g.TY * g.TZ);
+  const uint32_t fcap = m->cic_fcap ? m->cic_fcap
+                                     : (uint32_t) (CIC_FIXED_PER_CELL * g.TX * g.TY * g.TZ);
   if (m->cic_path == 0 && m->cic_fixed_ok && nk < (1ull << 27)) {
     const size_t need = nk * fcap;
     if (m->cic_list_cap < need) {
@@ -417,6 +421,22 @@ int cic_finish(Mesh *m) {
                               m->cic_pend.mom)) ||
         (e = sync_and_collect(m)))
       return e;
+    if (kind == 2) {
+      // ranges for the next fields of this instance: 1.25 x the fullest
+      // tile of this one, within the memory bound
+      const CicGeom g = cic_geometry(m->cic_pend.Ng, m->cic_pend.igs);
+      const size_t nk = cic_nkeys(g);
+      std::vector<uint32_t> cnt(nk);
+      HIP_OK(hipMemcpy(cnt.data(), m->cic_counts, nk * sizeof(uint32_t),
+          hipMemcpyDeviceToHost));
+      uint32_t mx = 0;
+      for (uint32_t c : cnt) mx = c > mx ? c : mx;
+      const uint32_t cap = ((mx + mx / 4) + 63) / 64 * 64;
+      if (cap <= (uint32_t) (CIC_FIXED_MAX_PER_CELL * g.TX * g.TY * g.TZ)) {
+        m->cic_fcap = cap;
+        m->cic_fixed_ok = true;
+      }
+    }
   }
   return 0;
 }

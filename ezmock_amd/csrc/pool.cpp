@@ -77,9 +77,9 @@ size_t EZmock_ext_footprint(const int Ngrid, const int with_powspec) {
   const size_t N = (size_t) Ngrid * Ngrid * Ngrid;
   const size_t s = sizeof(real);
   // psi x3, rho, vel4 (x4), rhot; nt (int16); CIC lists (fixed ranges of
-  // 4 u32 per cell); candidate keys and sort scratch (<= N keys worst case
-  // is covered by the head room of the estimate); catalogue
-  size_t b = N * s * 9 + 2 * N + 16 * N + N * sizeof(real) + (64u << 20);
+  // up to 6 u32 per cell); candidate keys and sort scratch (<= N keys worst
+  // case is covered by the head room of the estimate); catalogue
+  size_t b = N * s * 9 + 2 * N + 24 * N + N * sizeof(real) + (64u << 20);
   if (with_powspec) b += 2 * N * s;
   return b;
 }
