@@ -63,15 +63,6 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
     return table.get(kernel)
 
 
-def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
-
-
 def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=True):
     from ezmock_amd import EZmock, inputs
     k, P = inputs.eh_nowiggle_pk()
@@ -85,7 +76,6 @@ def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=T
         ntr = ez.populate_tracer_device(*params, nexp)
     ez.profile(profile)
     ez.reset_kernel_times()
-    sync()
     ez.sync()
     t0 = time.perf_counter()
     lat = []
@@ -95,7 +85,6 @@ def run_gpu(Ng, Lbox, seed, params, density, precision, steps, warmup, profile=T
         ntr = ez.populate_tracer_device(*params, nexp)
         lat.append(time.perf_counter() - t1)
     ez.sync()
-    sync()
     dt = time.perf_counter() - t0
     kt = ez.kernel_times()
     return dt, ntr, kt, lat
@@ -159,7 +148,7 @@ def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
             "mocks_per_s": round(nmock / dt, 2), "errors": bad[:3]}
 
 
-def run_serving(nreq=400, rate=20.0, nslot=4, budget=200e9, seed=7):
+def run_serving(nreq=2000, rate=20.0, nslot=4, budget=200e9, seed=7):
     """Serving loop: nreq requests arriving Poisson at `rate` per second,
     grids {128,192,256,384,512}^3 with weights {.3,.25,.2,.15,.1},
     Lbox = 3.9 Ng, bias parameters uniform in the calibration ranges,
@@ -206,11 +195,59 @@ def run_serving(nreq=400, rate=20.0, nslot=4, budget=200e9, seed=7):
             "latency_ms_max": round(float(lat.max()), 2), "slowest": slow}
 
 
+def host_threads():
+    """Host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when the environment sets it (a share of a larger host)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_dropin(steps=3, precision="f32"):
+    """The drop-in path: create_dens_field_from_ic + EZmock_populate_tracer,
+    which returns the catalogue in malloc'd host arrays (device-to-host copy
+    included), on the same 512^3 workload."""
+    from ezmock_amd import EZmock, inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = EZmock(Lbox=2500., Ngrid=512, rng=0, seed=42, precision=precision)
+    ez.eval_growth_params(0.5, 0.31)
+    ez.setup_linear_pk(k, P)
+    ez.create_dens_field_from_ic()
+    cat = ez.populate_tracer(0.8, 8., 0.25, 200., 3125000)
+    del cat
+    ez.sync()
+    t0 = time.perf_counter()
+    tp = 0.0
+    for _ in range(steps):
+        ez.create_dens_field_from_ic()
+        t1 = time.perf_counter()
+        cat = ez.populate_tracer(0.8, 8., 0.25, 200., 3125000)
+        tp += time.perf_counter() - t1
+        n = len(cat[0])
+        del cat
+    dt = time.perf_counter() - t0
+    return {"ms_per_mock": round(dt * 1e3 / steps, 2),
+            "populate_tracer_ms": round(tp * 1e3 / steps, 2), "tracers": n,
+            "host_bytes": n * 6 * (4 if precision == "f32" else 8)}
+
+
 def cpu_baseline(precision):
     """The oracle on the host cores, one mock of the same workload."""
     from oracle import pyoracle as po
     from ezmock_amd import inputs
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     k, P = inputs.eh_nowiggle_pk()
     Ng, L = 512, 2500.
     ez = po.EZmockOracle(Lbox=L, Ngrid=Ng, rng=0, seed=42, nthread=threads, prec=precision)
@@ -224,9 +261,11 @@ def cpu_baseline(precision):
     ez.close()
     return {"value": round(dt * 1e3, 1), "unit": "ms/mock", "cores": threads,
             "kind": "port",
-            "sample": f"one mock of the same workload (512^3, {precision}) by the oracle, a "
-                      f"C+OpenMP restatement of the reference CPU path with its own FFT, on "
-                      f"{threads} threads; stage seconds {stages}"}
+            "sample": f"one whole mock of the same workload (512^3, {precision}, not scaled) "
+                      f"by the oracle, a C+OpenMP restatement of the reference CPU path with "
+                      f"its own FFT, on {threads} OpenMP threads ({cpu_model()}, "
+                      f"{os.cpu_count()} logical CPUs on the host, {threads} available to this "
+                      f"process); stage seconds {stages}"}
 
 
 def main():
@@ -238,7 +277,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary lines (1024^3 FP32, 512^3 FP64, batch, serving)")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the serving trace (Poisson, 20 per second)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -296,6 +335,11 @@ def main():
         except Exception as e:
             extra["batch_64x256"] = f"failed: {e}"
         try:
+            extra["dropin_512_f32"] = run_dropin()
+            log(f"bench: drop-in {extra['dropin_512_f32']}")
+        except Exception as e:
+            extra["dropin_512_f32"] = f"failed: {e}"
+        try:
             extra["serving"] = run_serving(nreq=args.serve_requests)
             log(f"bench: serving {extra['serving']}")
         except Exception as e:
@@ -321,7 +365,7 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle sample ...")
         cpu = cpu_baseline(args.precision)
-        log(f"bench: cpu baseline {cpu['value']} ms/mock (scaled)")
+        log(f"bench: cpu baseline {cpu['value']} ms/mock on {cpu['cores']} threads")
     line = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/mock", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),

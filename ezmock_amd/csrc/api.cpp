@@ -72,6 +72,7 @@ struct Mesh {
   uint32_t *cic_ovf = nullptr, *cic_ovf_h = nullptr;
   bool cic_fixed_ok = true;
   uint32_t cic_fcap = 0;        // entries per tile range (0: the default)
+  size_t cic_list_limit = 0;    // bytes allowed for fixed ranges (0: no limit)
   // 0: binning, fixed ranges (exact after an overflow); 1: binning, exact
   // ranges; 2: tile scan where valid
   int cic_path = 0;
@@ -310,7 +311,16 @@ int paint_cic_binned(Mesh *m, int Ng, real igs, real *rho, bool with_mom) {
     const size_t need = nk * fcap;
     if (m->cic_list_cap < need) {
       dfree(&m->cic_list);
-      if ((e = dalloc(&m->cic_list, need))) { m->cic_list_cap = 0; return e; }
+      m->cic_list_cap = 0;
+      // fixed ranges need more memory than exact ones (about 1.35 N
+      // entries); where they do not fit (allocation failure or the
+      // instance's list limit), this and later fields use exact ranges
+      if ((m->cic_list_limit && need * sizeof(uint32_t) > m->cic_list_limit) ||
+          dalloc(&m->cic_list, need)) {
+        m->cic_fixed_ok = false;
+        m->cic_fcap = 0;
+        return paint_cic_binned(m, Ng, igs, rho, with_mom);
+      }
       m->cic_list_cap = need;
     }
     if ((e = dalloc(&m->cic_ovf, 1))) return e;
@@ -1397,6 +1407,11 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
     m->cic_fixed_ok = true;
   }
   else if (option == 4) m->fft_streams = value != 0;
+  else if (option == 5 && value >= 0) {
+    m->cic_list_limit = (size_t) value;
+    m->cic_fixed_ok = true;
+    m->cic_fcap = 0;
+  }
   else return EZMOCK_ERR_ARG_PARAMS;
   return EZMOCK_SUCCESS;
 }

@@ -56,14 +56,20 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
 /* Options: 1 = keep a copy of the generated spectrum (tests);
             2 = record per-kernel device times with HIP events;
             3 = CIC method: 0 = particle binning into fixed list ranges
-                per tile (default; exact ranges for the instance after an
-                overflow), 1 = particle binning with exact ranges (count
-                pass), 2 = tile scan of the displaced lattice wherever the
-                displacement range admits it (no particle lists), else
-                binning.  All give the same fields;
+                per tile (default: 4 entries per cell; after an overflow
+                the field is repainted with exact ranges and the ranges of
+                later fields become 1.25 x the fullest tile, if that is at
+                most 6 per cell, else exact ranges), 1 = particle binning
+                with exact ranges (count pass), 2 = tile scan of the
+                displaced lattice wherever the displacement range admits it
+                (no particle lists), else binning.  All give the same
+                fields;
             4 = run the last two FFT passes of the three displacement
                 components on three streams (1, default) or on one (0);
-                one stream while option 2 is on. */
+                one stream while option 2 is on;
+            5 = bytes of device memory the fixed list ranges may take
+                (0 = no limit, the default); above it, or when their
+                allocation fails, exact ranges are used. */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset

@@ -6,6 +6,7 @@ velocity scatter, and run-to-run determinism of the integer stages."""
 import numpy as np
 import pytest
 
+import pincheck as pc
 from ezmock_amd import EZmock
 
 pytestmark = pytest.mark.gpu
@@ -79,8 +80,8 @@ def test_integer_stages_deterministic(gpu_available, mock512):
 def test_cic_tile_scan_matches_binning(gpu_available, pk_table, mock512):
     """The tile-scan CIC option on the same 512^3 field: the density
     identical in > 99.9% of the cells (double sums in another order,
-    rounded once), the tracers per cell and the catalogue identical when
-    every cell agrees."""
+    rounded once), >= 99.99% of the tracers identical (matched by host cell
+    and rank within the cell)."""
     g, n = mock512
     k, P = pk_table
     h = EZmock(Lbox=2500., Ngrid=512, rng=0, seed=42, precision="f32")
@@ -94,8 +95,9 @@ def test_cic_tile_scan_matches_binning(gpu_available, pk_table, mock512):
     assert np.mean(ra == rb) > 0.999
     assert np.abs(ra.astype(np.float64) - rb).max() <= 1e-6 * ra.max()
     n2 = h.populate_tracer_device(0.8, 8., 0.25, 200., 3125000)
-    same = np.mean(g.field("nt") == h.field("nt"))
-    assert same >= 0.9999
-    if same == 1.0:
-        assert n2 == n
-        assert np.array_equal(g.device_catalog()[0], h.device_catalog()[0])
+    assert np.mean(g.field("nt") == h.field("nt")) >= 0.9999
+    assert abs(n2 - n) <= 1e-4 * n
+    frac = pc.identical_tracer_fraction(g.field("nt"), g.device_catalog(), h.field("nt"),
+                                        h.device_catalog(), 2e-5)
+    print(f"tile scan vs binning: identical tracers {frac:.6f} of {n}")
+    assert frac >= 0.9999

@@ -4,12 +4,15 @@ oracle on the same seeded inputs, and against numpy / closed forms.
 Tolerances (stated per check): integer, index and stream work is bit-exact;
 spectra are bit-exact in FP32 and within a few ulp in FP64 (device vs glibc
 transcendentals); FFT outputs within FP rounding; the FP64 density field
-within 1e-10 relative and >= 99.99% identical tracer cells."""
+within 1e-10 relative, >= 99.99% identical tracer cells and >= 99.99%
+identical tracers; FP32 tracers identical up to the first cell whose count
+differs from the oracle's."""
 import os
 
 import numpy as np
 import pytest
 
+import pincheck as pc
 from ezmock_amd import EZmock, uniform, fft_c2r, fft_r2c, pdf_walk
 from oracle import pyoracle as po
 
@@ -146,16 +149,18 @@ def test_pipeline_parity_small(gpu_available, pk_table, Ng, prec, fixamp, iphase
     pg, pO = g.pdf_info(), o.pdf_info()
     assert pg["nmax"] == pO["nmax"]
     ntg, nto = g.field("nt"), o.field("nt")
-    same = np.mean(ntg == nto)
-    assert same >= 0.9999
-    if same == 1.0:
-        assert (pg["ncell"], pg["ntot"]) == (pO["ncell"], pO["ntot"])
-        assert np.array_equal(pg["pdf"], pO["pdf"])
-        for a, b in zip(cg[:3], co[:3]):
-            assert np.array_equal(a, b)              # positions bit-exact
-        vt = 1e-9 if prec == "f64" else 2e-5
-        for a, b in zip(cg[3:], co[3:]):
-            assert np.abs(a.astype(float) - b).max() <= vt * max(1.0, np.abs(b).max())
+    assert np.mean(ntg == nto) >= 0.9999
+    assert abs(pg["ntot"] - pO["ntot"]) <= 1e-4 * pO["ntot"]
+    # identical tracers (positions bit-equal, velocities within vt of the
+    # largest speed), matched by host cell and rank within the cell; in FP32
+    # up to the first cell whose count differs (float sums of the oracle)
+    vt = 1e-9 if prec == "f64" else 2e-5
+    frac = pc.identical_tracer_fraction(ntg, cg, nto, co, vt)
+    print(f"identical tracers {frac:.6f} of {len(co[0])}")
+    if prec == "f64":
+        assert frac >= 0.9999
+    else:
+        pc.identical_before_first_difference(ntg, cg, nto, co, vt)
 
 
 def test_white_noise_and_displacement_inputs(gpu_available, pk_table):
@@ -237,15 +242,13 @@ def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
     rg, ro = g.field("rho"), o.field("rho")
     assert np.abs(rg - ro).max() <= 1e-10 * np.abs(ro).max()
     ntg, nto = g.field("nt"), o.field("nt")
-    same = np.mean(ntg == nto)
-    assert same >= 0.9999
+    assert np.mean(ntg == nto) >= 0.9999
     pg, pO = g.pdf_info(), o.pdf_info()
-    assert abs(pg["ntot"] - pO["ntot"]) <= max(3, 1e-4 * pO["ntot"])
-    if same == 1.0:
-        for a, b in zip(cg[:3], co[:3]):
-            assert np.array_equal(a, b)
-        for a, b in zip(cg[3:], co[3:]):
-            assert np.abs(a - b).max() < 1e-6
+    assert abs(pg["ntot"] - pO["ntot"]) <= 1e-4 * pO["ntot"]
+    frac = pc.identical_tracer_fraction(ntg, cg, nto, co, 1e-9)
+    print(f"config 1: identical tracers {frac:.6f} of {len(co[0])}, ntot {pg['ntot']} "
+          f"vs {pO['ntot']}")
+    assert frac >= 0.9999
 
 
 def test_fft_component_streams_same_fields(gpu_available, pk_table):
