@@ -359,6 +359,25 @@ def fft_r2c(x, precision="f32"):
     return out[..., 0] + 1j * out[..., 1]
 
 
+def fft_check(Ng, path, modes, amps, precision="f32"):
+    """Plane waves through the GPU c2r passes, checked on the device
+    (EZmock_ext_fft_check): path 0 = generic passes (one output), path 1 =
+    production first pass of the three displacement components.  modes:
+    [(i, j, kk), ...] with 0 < kk < Ng/2; amps: complex amplitudes.
+    Returns (max abs error, max abs closed form) per output."""
+    L = _lib.load(precision)
+    nout = 3 if path else 1
+    ijk = np.ascontiguousarray(np.asarray(modes, dtype=np.int32).reshape(-1))
+    a = np.ascontiguousarray(np.stack([np.real(amps), np.imag(amps)], -1).astype(np.float64))
+    err_, ref = np.zeros(nout), np.zeros(nout)
+    err = C.c_int(0)
+    L.EZmock_ext_fft_check(int(Ng), int(path), len(modes), ijk.ctypes.data, a.ctypes.data,
+                           err_.ctypes.data, ref.ctypes.data, C.byref(err))
+    if err.value:
+        raise RuntimeError(L.EZmock_errmsg(err.value).decode())
+    return err_, ref
+
+
 def pdf_walk(rhot, ncell, pdf, precision="f32"):
     """GPU PDF walk on an arbitrary tracer-density array."""
     L = _lib.load(precision)

@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -101,10 +102,24 @@ struct Mesh {
   // MT19937 stream words of the current stage (two per draw)
   uint32_t *mtw = nullptr;
   size_t mtw_cap = 0;
+  // tracers attached to particles: particles per cell, particle classes,
+  // tracers left per cell, block counts / offsets, contested cells and
+  // particles, their selection and draws
+  uint32_t *att_np = nullptr, *att_bcnt = nullptr, *att_cinfo = nullptr,
+      *att_plist = nullptr, *att_ovf = nullptr;
+  uint8_t *att_cls = nullptr, *att_sel = nullptr;
+  int16_t *att_rest = nullptr;
+  uint64_t *att_boff[3] = {nullptr, nullptr, nullptr}, *att_stmp = nullptr;
+  double *att_draws = nullptr;
+  size_t att_ccap = 0, att_pcap = 0;
   cplx *pk_ny = nullptr;
   double *pk_acc = nullptr;
   real *cat[6] = {nullptr};
-  size_t cat_cap = 0, ntracer = 0;
+  size_t cat_cap = 0, ntracer = 0, natt = 0;
+  // pinned host staging of the catalogue for EZmock_populate_tracer
+  real *pin = nullptr;
+  size_t pin_cap = 0;           // tracers per array
+  hipEvent_t ev_pos = nullptr, ev_xyz = nullptr;
   // last PDF
   size_t nmax = 0, ncell = 0, ntot = 0, ncell0 = 0, ntot0 = 0;
   std::vector<size_t> pdf;
@@ -144,6 +159,8 @@ int device_ready(Mesh *m) {
     HIP_OK(hipEventCreateWithFlags(&m->ev_join[i], hipEventDisableTiming));
   }
   HIP_OK(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&m->ev_pos, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&m->ev_xyz, hipEventDisableTiming));
   {
     std::lock_guard<std::mutex> lk(g_jtab_mu);
     auto it = g_jtab.find(dev);
@@ -200,6 +217,11 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->d_rsd);
   dfree(&m->pk_grid[0]); dfree(&m->pk_grid[1]); dfree(&m->pk_ny); dfree(&m->pk_acc);
   dfree(&m->mtw); m->mtw_cap = 0;
+  dfree(&m->att_np); dfree(&m->att_bcnt); dfree(&m->att_cinfo); dfree(&m->att_plist);
+  dfree(&m->att_ovf); dfree(&m->att_cls); dfree(&m->att_sel); dfree(&m->att_rest);
+  for (int i = 0; i < 3; i++) dfree(&m->att_boff[i]);
+  dfree(&m->att_stmp); dfree(&m->att_draws);
+  m->att_ccap = m->att_pcap = 0;
   for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
   m->cat_cap = 0;
   for (auto &t : m->pending) { (void) hipEventDestroy(t.a); (void) hipEventDestroy(t.b); }
@@ -214,6 +236,12 @@ void free_mesh_device(Mesh *m) {
   }
   if (m->ev_fork) (void) hipEventDestroy(m->ev_fork);
   m->ev_fork = nullptr;
+  if (m->ev_pos) (void) hipEventDestroy(m->ev_pos);
+  if (m->ev_xyz) (void) hipEventDestroy(m->ev_xyz);
+  m->ev_pos = m->ev_xyz = nullptr;
+  if (m->pin) (void) hipHostFree(m->pin);
+  m->pin = nullptr;
+  m->pin_cap = 0;
 }
 
 // Kernel-class timing with HIP events on the instance stream (option 2).
@@ -545,9 +573,202 @@ int ensure_cat(Mesh *m, size_t n) {
   return 0;
 }
 
+// Exclusive scan of per-block u32 counts into out; returns the total.
+int scan_total(Mesh *m, const uint32_t *in, uint64_t *out, size_t nblk, uint64_t *tmp,
+    uint64_t *total) {
+  KCHK(launch_scan_u32(in, out, nblk, tmp, m->st));
+  const size_t nch = (nblk + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  HIP_OK(hipMemcpyAsync(total, tmp + nch, sizeof(uint64_t), hipMemcpyDeviceToHost, m->st));
+  return sync_and_collect(m);
+}
+
+// Tracers attached to particles (att_part, pop_tracer.c:517-582 and
+// 804-879), written to the front of the catalogue; *iv = their number, and
+// *rest / *restoff the tracers per cell left for uniform positions and
+// their block offsets.  The particle counts, classes and the writing of the
+// catalogue are parallel passes; the selection itself is the reference's
+// serial, stream-ordered Bentley selection, run on the host over the
+// particles of contested cells only (cells with more particles than
+// tracers; the others take all their particles without a draw).  A cell
+// keeps max(nt - np, 0) tracers for uniform positions whatever the draws.
+int attach_particles(EZMOCK *ez, Mesh *m, size_t ntot, size_t *iv,
+    const int16_t **rest, const uint64_t **restoff) {
+  Conf *conf = CONF(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const size_t nblk = (N + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  AttGeom ag;
+  ag.Ng = Ng;
+  ag.igs = conf->Ng / conf->Lbox;
+  ag.gs = conf->Lbox / conf->Ng;
+  ag.N = N;
+  ag.fNg = fastdiv_make((uint32_t) Ng);
+  int e;
+  if ((e = dalloc(&m->att_np, N)) || (e = dalloc(&m->att_cls, N)) ||
+      (e = dalloc(&m->att_rest, N)) || (e = dalloc(&m->att_bcnt, nblk)) ||
+      (e = dalloc(&m->att_boff[0], nblk)) || (e = dalloc(&m->att_boff[1], nblk)) ||
+      (e = dalloc(&m->att_boff[2], nblk)) ||
+      (e = dalloc(&m->att_stmp, scan_tmp_words(nblk) + 8)) || (e = dalloc(&m->att_ovf, 1)))
+    return e;
+  // particles per tracer cell, with the reference's uint16 limit
+  uint32_t ovf = 0;
+  {
+    Scope s(m, "att_count");
+    KCHK(launch_zero(m->att_np, N * sizeof(uint32_t), m->st));
+    KCHK(launch_zero(m->att_ovf, sizeof(uint32_t), m->st));
+    KCHK(launch_att_count(m->psi[0], m->psi[1], m->psi[2], ag, m->nt, m->att_np, m->att_ovf,
+        m->st));
+  }
+  HIP_OK(hipMemcpyAsync(&ovf, m->att_ovf, sizeof(uint32_t), hipMemcpyDeviceToHost, m->st));
+  if ((e = sync_and_collect(m))) return e;
+  if (ovf) return EZMOCK_ERR_RHO_NPART;
+  // contested cells and particles
+  uint64_t ncc = 0, ncont = 0;
+  {
+    Scope s(m, "att_cells");
+    KCHK(launch_att_cells(m->nt, m->att_np, N, m->att_bcnt, nullptr, nullptr, m->st));
+  }
+  if ((e = scan_total(m, m->att_bcnt, m->att_boff[0], nblk, m->att_stmp, &ncc))) return e;
+  if (m->att_ccap < ncc + 1) {
+    dfree(&m->att_cinfo);
+    m->att_ccap = 0;
+    if ((e = dalloc(&m->att_cinfo, ncc + 1))) return e;
+    m->att_ccap = ncc + 1;
+  }
+  {
+    Scope s(m, "att_cells");
+    KCHK(launch_att_cells(m->nt, m->att_np, N, m->att_bcnt, m->att_boff[0], m->att_cinfo,
+        m->st));
+  }
+  {
+    Scope s(m, "att_classify");
+    KCHK(launch_att_classify(m->psi[0], m->psi[1], m->psi[2], ag, m->nt, m->att_np,
+        m->att_cls, m->att_bcnt, m->st));
+  }
+  if ((e = scan_total(m, m->att_bcnt, m->att_boff[1], nblk, m->att_stmp, &ncont))) return e;
+  std::vector<uint8_t> selh(ncont);
+  if (ncont) {
+    if (m->att_pcap < ncont) {
+      dfree(&m->att_plist); dfree(&m->att_sel); dfree(&m->att_draws);
+      m->att_pcap = 0;
+      if ((e = dalloc(&m->att_plist, ncont)) || (e = dalloc(&m->att_sel, ncont)) ||
+          (e = dalloc(&m->att_draws, ncont)))
+        return e;
+      m->att_pcap = ncont;
+    }
+    {
+      Scope s(m, "att_list");
+      KCHK(launch_att_list(m->psi[0], m->psi[1], m->psi[2], ag, m->att_np, m->att_cls,
+          m->att_boff[1], m->att_plist, m->st));
+    }
+    // at most one draw per contested particle, from stream position 4N
+    // (pop_tracer.c:781, 846)
+    if (RNG(ez)->type == 1) {
+      const uint32_t *w = nullptr;
+      if ((e = mt_words(ez, m, 4 * (uint64_t) N, ncont, &w))) return e;
+      KCHK(launch_uniform_mt(w, ncont, 0, m->att_draws, m->st));
+    }
+    else {
+      MrgState st0;
+      mrg_seed_state(RNG(ez)->seed, st0);
+      KCHK(launch_uniform(st0, 4 * (uint64_t) N, ncont, m->jtab, m->att_draws, m->st));
+    }
+    std::vector<uint32_t> pl(ncont), ci(ncc);
+    std::vector<double> dr(ncont);
+    HIP_OK(hipMemcpyAsync(pl.data(), m->att_plist, ncont * sizeof(uint32_t),
+        hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipMemcpyAsync(ci.data(), m->att_cinfo, ncc * sizeof(uint32_t),
+        hipMemcpyDeviceToHost, m->st));
+    HIP_OK(hipMemcpyAsync(dr.data(), m->att_draws, ncont * sizeof(double),
+        hipMemcpyDeviceToHost, m->st));
+    if ((e = sync_and_collect(m))) return e;
+    // pop_tracer.c:841-854, in Lagrangian order
+    std::vector<uint16_t> tt(ncc), nn(ncc);
+    for (size_t c = 0; c < ncc; c++) {
+      tt[c] = (uint16_t) (ci[c] & 0xffffu);
+      nn[c] = (uint16_t) (ci[c] >> 16);
+    }
+    size_t d = 0;
+    for (size_t k = 0; k < ncont; k++) {
+      const uint32_t c = pl[k];
+      uint8_t take = 0;
+      if (tt[c] != 0) {
+        bool skip = false;
+        if (tt[c] < nn[c]) {
+          const double ran = dr[d++];
+          if (ran * nn[c] >= tt[c]) {
+            nn[c]--;
+            skip = true;
+          }
+        }
+        if (!skip) {
+          tt[c]--;
+          nn[c]--;
+          take = 1;
+        }
+      }
+      selh[k] = take;
+    }
+    HIP_OK(hipMemcpyAsync(m->att_sel, selh.data(), ncont, hipMemcpyHostToDevice, m->st));
+  }
+  // the attached tracers at the front of the catalogue, in Lagrangian order
+  uint64_t nsel = 0, nrest = 0;
+  real *const cat[6] = {m->cat[0], m->cat[1], m->cat[2], m->cat[3], m->cat[4], m->cat[5]};
+  {
+    Scope s(m, "att_select");
+    KCHK(launch_att_select(m->psi[0], m->psi[1], m->psi[2], ag, COSMO(ez)->vfac, m->att_cls,
+        m->att_boff[1], m->att_sel, m->att_bcnt, nullptr, cat, m->st));
+  }
+  if ((e = scan_total(m, m->att_bcnt, m->att_boff[0], nblk, m->att_stmp, &nsel))) return e;
+  if (nsel > ntot) return EZMOCK_ERR_UNKNOWN;
+  {
+    Scope s(m, "att_select");
+    KCHK(launch_att_select(m->psi[0], m->psi[1], m->psi[2], ag, COSMO(ez)->vfac, m->att_cls,
+        m->att_boff[1], m->att_sel, nullptr, m->att_boff[0], cat, m->st));
+    KCHK(launch_att_rest(m->nt, m->att_np, N, m->att_rest, m->att_bcnt, m->st));
+  }
+  if ((e = scan_total(m, m->att_bcnt, m->att_boff[2], nblk, m->att_stmp, &nrest))) return e;
+  if (nsel + nrest != ntot) return EZMOCK_ERR_UNKNOWN;   // pop_tracer.c:989-993
+  *iv = nsel;
+  *rest = m->att_rest;
+  *restoff = m->att_boff[2];
+  return 0;
+}
+
+// Catalogue arrays for the caller (EZmock_populate_tracer): malloc'd host
+// arrays filled from pinned staging buffers.
+struct HostCat {
+  real *h[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void release() {
+    for (int i = 0; i < 6; i++) { free(h[i]); h[i] = nullptr; }
+  }
+};
+
+// dst[a][0, n) = src[a][0, n) for arrays a in [a0, a1), in pieces of 1 M
+// elements spread over up to nthread (at most 8) host threads.
+void copy_arrays(real *const *dst, real *const *src, int a0, int a1, size_t n, int nthread) {
+  const size_t CH = (size_t) 1 << 20;
+  const size_t per = (n + CH - 1) / CH;              // pieces per array
+  const size_t npiece = per * (size_t) (a1 - a0);
+  int nt = nthread < 1 ? 1 : (nthread > 8 ? 8 : nthread);
+  if ((size_t) nt > npiece) nt = npiece ? (int) npiece : 1;
+  auto work = [&](int t) {
+    for (size_t q = (size_t) t; q < npiece; q += (size_t) nt) {
+      const int a = a0 + (int) (q / per);
+      const size_t off = (q % per) * CH;
+      const size_t len = n - off < CH ? n - off : CH;
+      memcpy(dst[a] + off, src[a] + off, len * sizeof(real));
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+}
+
 // The device part of EZmock_populate_tracer; catalogue left in m->cat.
 int populate_device(EZMOCK *ez, const real *params, size_t nexp,
-    bool att_part, size_t *ntracer) {
+    bool att_part, size_t *ntracer, HostCat *hc = nullptr) {
   Conf *conf = CONF(ez);
   Mesh *m = MESH(ez);
   Cosmo *cosmo = COSMO(ez);
@@ -568,11 +789,6 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   m->ncell0 = ncell;
   m->ntot0 = ntot;
 
-  if (att_part) {
-    fprintf(stderr, "EZmock: att_part = true (attach tracers to particles) "
-        "is not available on the GPU engine\n");
-    return EZMOCK_ERR_UNKNOWN;
-  }
   if ((e = device_ready(m))) return e;
   if (m->psi_ref) {             // the reference reads the caller's arrays
     for (int i = 0; i < 3; i++)
@@ -700,12 +916,48 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if ((e = ensure_cat(m, ntot))) return e;
   const real gs = conf->Lbox / conf->Ng;
   const real igs = conf->Ng / conf->Lbox;
+  // tracers attached to particles first (att_part), then the others at
+  // uniform positions in their cells, pop_tracer.c:881-1007
+  size_t iv = 0;
+  const int16_t *ntpos = m->nt;
+  const uint64_t *offpos = m->blockoff;
+  if (att_part && (e = attach_particles(ez, m, ntot, &iv, &ntpos, &offpos))) return e;
   const uint32_t *mtw_pos = nullptr;
-  if ((e = mt_words(ez, m, 5 * (uint64_t) N, 3 * (uint64_t) ntot, &mtw_pos))) return e;
-  {
+  if ((e = mt_words(ez, m, 5 * (uint64_t) N, 3 * (uint64_t) (ntot - iv), &mtw_pos))) return e;
+  if (ntot > iv) {
     Scope s(m, "positions");
-    KCHK(launch_positions(m->nt, m->blockoff, Ng, conf->Lbox, gs, bp.seed_state,
-        m->jtab, mtw_pos, m->cat[0], m->cat[1], m->cat[2], ntot, m->st));
+    KCHK(launch_positions(ntpos, offpos, Ng, conf->Lbox, gs, bp.seed_state,
+        m->jtab, mtw_pos, m->cat[0] + iv, m->cat[1] + iv, m->cat[2] + iv, ntot - iv, m->st));
+  }
+  m->natt = iv;
+  // host catalogue: the positions go to pinned staging on a second stream
+  // while the velocities are computed
+  real *stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (hc) {
+    if (m->pin_cap < ntot) {
+      if (m->pin) (void) hipHostFree(m->pin);
+      m->pin = nullptr;
+      m->pin_cap = 0;
+      const size_t cap = ntot + ntot / 8 + 1024;
+      if (hipHostMalloc((void **) &m->pin, 6 * cap * sizeof(real)) != hipSuccess) {
+        m->pin = nullptr;
+        return EZMOCK_ERR_MEMORY;
+      }
+      m->pin_cap = cap;
+    }
+    for (int i = 0; i < 6; i++) {
+      stage[i] = m->pin + (size_t) i * m->pin_cap;
+      if (!(hc->h[i] = (real *) malloc((ntot ? ntot : 1) * sizeof(real)))) {
+        hc->release();
+        return EZMOCK_ERR_MEMORY;
+      }
+    }
+    HIP_OK(hipEventRecord(m->ev_pos, m->st));
+    HIP_OK(hipStreamWaitEvent(m->sx[0], m->ev_pos, 0));
+    for (int i = 0; i < 3; i++)
+      HIP_OK(hipMemcpyAsync(stage[i], m->cat[i], ntot * sizeof(real), hipMemcpyDeviceToHost,
+          m->sx[0]));
+    HIP_OK(hipEventRecord(m->ev_xyz, m->sx[0]));
   }
   // ---- velocities, pop_tracer.c:593-753 and :1043-1113
   if (!m->mom_valid) {
@@ -720,10 +972,23 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if ((e = mt_words(ez, m, 8 * (uint64_t) N, 6 * (((uint64_t) ntot + 1) / 2), &mtw_vel))) return e;
   {
     Scope s(m, "velocities");
-    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, Ng, igs, cosmo->vfac, (double) sigma_v,
+    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, iv, Ng, igs,
+        cosmo->vfac, (double) sigma_v,
         bp.seed_state, m->jtab, mtw_vel, m->cat[3], m->cat[4], m->cat[5], m->st));
   }
-  if ((e = sync_and_collect(m))) return e;
+  if (hc) {
+    for (int i = 3; i < 6; i++)
+      HIP_OK(hipMemcpyAsync(stage[i], m->cat[i], ntot * sizeof(real), hipMemcpyDeviceToHost,
+          m->st));
+    // positions into the caller's arrays while the velocities finish
+    if (hipEventSynchronize(m->ev_xyz) != hipSuccess) { hc->release(); return EZMOCK_ERR_UNKNOWN; }
+    copy_arrays(hc->h, stage, 0, 3, ntot, conf->nthread);
+  }
+  if ((e = sync_and_collect(m))) {
+    if (hc) hc->release();
+    return e;
+  }
+  if (hc) copy_arrays(hc->h, stage, 3, 6, ntot, conf->nthread);
   m->ntracer = ntot;
   *ntracer = ntot;
   Par *par = PAR(ez);
@@ -1115,29 +1380,10 @@ int EZmock_populate_tracer(EZMOCK *ez, const real *params, const size_t nexp,
   int e = populate_checks(ez, params, nexp, ntracer);
   if (e) return (*err = e);
   size_t n = 0;
-  if ((e = populate_device(ez, params, nexp, att_part, &n))) return (*err = e);
-  Mesh *m = MESH(ez);
+  HostCat hc;
+  if ((e = populate_device(ez, params, nexp, att_part, &n, &hc))) return (*err = e);
   real **out[6] = {x, y, z, vx, vy, vz};
-  real *h[6] = {nullptr};
-  for (int i = 0; i < 6; i++) {
-    h[i] = (real *) malloc((n ? n : 1) * sizeof(real));
-    if (!h[i]) {
-      for (int j = 0; j < i; j++) free(h[j]);
-      return (*err = EZMOCK_ERR_MEMORY);
-    }
-  }
-  for (int i = 0; i < 6; i++) {
-    if (hipMemcpyAsync(h[i], m->cat[i], n * sizeof(real), hipMemcpyDeviceToHost,
-        m->st) != hipSuccess) {
-      for (int j = 0; j < 6; j++) free(h[j]);
-      return (*err = EZMOCK_ERR_UNKNOWN);
-    }
-  }
-  if (hipStreamSynchronize(m->st) != hipSuccess) {
-    for (int j = 0; j < 6; j++) free(h[j]);
-    return (*err = EZMOCK_ERR_UNKNOWN);
-  }
-  for (int i = 0; i < 6; i++) *out[i] = h[i];
+  for (int i = 0; i < 6; i++) *out[i] = hc.h[i];
   *ntracer = n;
   return EZMOCK_SUCCESS;
 }
@@ -1581,6 +1827,97 @@ static int fft_test(const int Ng, const real *in, real *out, int sign, int *err)
     }
   }
   dfree(&buf); dfree(&half);
+  free_mesh_device(&tmp);
+  return (*err = e);
+}
+
+// Plane waves through the c2r passes at any supported size, checked on the
+// device against their closed form (no host copies of the fields).
+int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const int *ijk,
+    const double *amp, double *maxerr, double *maxref, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err) return *err;
+  if (!fft_supported(Ng)) return (*err = EZMOCK_ERR_ARG_NGRID);
+  if ((path != 0 && path != 1) || nmodes < 1 || nmodes > EZ_FFT_MAX_MODES || !ijk || !amp ||
+      !maxerr || !maxref)
+    return (*err = EZMOCK_ERR_ARG_NULL);
+  const int Ngh = Ng / 2;
+  FftModes md;
+  md.n = nmodes;
+  for (int m = 0; m < nmodes; m++) {
+    const int i0 = ijk[3 * m], j0 = ijk[3 * m + 1], k0 = ijk[3 * m + 2];
+    if (i0 < 0 || i0 >= Ng || j0 < 0 || j0 >= Ng || k0 <= 0 || k0 >= Ngh)
+      return (*err = EZMOCK_ERR_ARG_NULL);
+    for (int q = 0; q < m; q++)
+      if (md.ijk[q][0] == i0 && md.ijk[q][1] == j0 && md.ijk[q][2] == k0)
+        return (*err = EZMOCK_ERR_ARG_NULL);
+    md.ijk[m][0] = i0; md.ijk[m][1] = j0; md.ijk[m][2] = k0;
+    md.re[m] = amp[2 * m];
+    md.im[m] = amp[2 * m + 1];
+  }
+  Mesh tmp;
+  int e = device_ready(&tmp);
+  if (e) return (*err = e);
+  const size_t N = (size_t) Ng * Ng * Ng, plane = (size_t) Ng * Ng;
+  const int nout = path ? 3 : 1;
+  real *out[3] = {nullptr, nullptr, nullptr};
+  unsigned long long *res = nullptr;
+  e = 0;
+  for (int c = 0; c < nout && !e; c++) e = dalloc(&out[c], N);
+  if (!e && ((e = dalloc(&tmp.rho, N)) || (e = dalloc(&tmp.ny, plane)) ||
+      (e = dalloc(&res, 2 * nout)) || (e = ensure_twiddles(&tmp, Ng)) ||
+      (path && (e = dalloc(&tmp.slot0, 3 * plane)))))
+    ;
+  if (!e) {
+    cplx *rhok = (cplx *) tmp.rho;
+    if (launch_zero(tmp.rho, N * sizeof(real), tmp.st) != hipSuccess ||
+        launch_zero(tmp.ny, plane * sizeof(cplx), tmp.st) != hipSuccess ||
+        launch_zero(res, 2 * nout * sizeof(unsigned long long), tmp.st) != hipSuccess)
+      e = EZMOCK_ERR_UNKNOWN;
+    for (int m = 0; m < nmodes && !e; m++) {
+      const cplx a = cplx{(real) md.re[m], (real) md.im[m]};
+      md.re[m] = a.x;                       // the closed form uses the stored values
+      md.im[m] = a.y;
+      const size_t at = ((size_t) md.ijk[m][0] * Ng + md.ijk[m][1]) * Ngh + md.ijk[m][2];
+      if (hipMemcpyAsync(rhok + at, &a, sizeof(cplx), hipMemcpyHostToDevice, tmp.st) != hipSuccess ||
+          hipStreamSynchronize(tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+    }
+    if (!e && path == 0) {
+      cplx *b = (cplx *) out[0];
+      if (fft_strided_pass(b, rhok, tmp.ny, tmp.twN, Ng, 0, +1, 1, 3, tmp.st) != hipSuccess ||
+          fft_strided_pass(b, nullptr, nullptr, tmp.twN, Ng, 1, +1, 0, 0, tmp.st) != hipSuccess ||
+          fft_row_pass(b, tmp.twN, tmp.twh, Ng, +1, tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+    }
+    else if (!e) {
+      if (fft_slot0_pack(rhok, tmp.ny, tmp.slot0, Ng, tmp.st) != hipSuccess ||
+          fft_first_pass3((cplx *) out[0], (cplx *) out[1], (cplx *) out[2], rhok, tmp.slot0,
+              tmp.twN, Ng, tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+      for (int c = 0; c < 3 && !e; c++) e = c2r_finish(&tmp, Ng, out[c], tmp.st);
+    }
+    for (int c = 0; c < nout && !e; c++) {
+      FftModes mc = md;
+      for (int m = 0; m < nmodes; m++) {
+        const int n = c == 2 ? md.ijk[m][2] : md.ijk[m][c];
+        mc.w[m] = path == 0 ? 1.0 : (double) ((c < 2 && n > Ngh) ? n - Ng : n);
+      }
+      if (launch_plane_wave_check(out[c], Ng, mc, res + 2 * c, tmp.st) != hipSuccess)
+        e = EZMOCK_ERR_UNKNOWN;
+    }
+    unsigned long long rh[6] = {0};
+    if (!e && (hipMemcpyAsync(rh, res, 2 * nout * sizeof(unsigned long long),
+        hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
+        hipStreamSynchronize(tmp.st) != hipSuccess))
+      e = EZMOCK_ERR_UNKNOWN;
+    for (int c = 0; c < nout && !e; c++) {
+      memcpy(&maxerr[c], &rh[2 * c], sizeof(double));
+      memcpy(&maxref[c], &rh[2 * c + 1], sizeof(double));
+    }
+  }
+  for (int c = 0; c < 3; c++) dfree(&out[c]);
+  dfree(&res);
   free_mesh_device(&tmp);
   return (*err = e);
 }

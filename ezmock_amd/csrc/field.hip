@@ -289,3 +289,48 @@ hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_scale, dim3(ez_blocks(n, 256)), dim3(256), 0, st, p, s, n);
   return hipGetLastError();
 }
+
+// ------------------------------------------------ plane-wave FFT checks
+// Largest |out - expected| and |expected| over a real field that a c2r
+// transform produced from a spectrum of a few modes (i0, j0, kk0) with
+// 0 < kk0 < Ng/2: expected(x, y, z) = sum_m 2 w_m Re(A_m e^{i theta_m}),
+// theta_m = 2 pi ((i0 x + j0 y + kk0 z) mod Ng) / Ng, evaluated in double
+// with the phase reduced exactly in integers.  res[0], res[1]: bit patterns
+// of the two maxima (non-negative doubles order as unsigned integers);
+// cleared by the caller.  One workgroup per (x, y) row.
+__global__ void __launch_bounds__(256)
+k_plane_wave_check(const real *__restrict__ out, int Ng, FftModes md,
+    unsigned long long *__restrict__ res) {
+  const size_t row = blockIdx.x;
+  const int x = (int) (row / (size_t) Ng), y = (int) (row % (size_t) Ng);
+  double emax = 0, rmax = 0;
+  for (int z = threadIdx.x; z < Ng; z += blockDim.x) {
+    double e = 0;
+    for (int m = 0; m < md.n; m++) {
+      const long long ph = ((long long) md.ijk[m][0] * x + (long long) md.ijk[m][1] * y +
+                            (long long) md.ijk[m][2] * z) % Ng;
+      double sn, cs;
+      sincos(2 * EZ_PI * (double) ph / Ng, &sn, &cs);
+      e += 2 * md.w[m] * (md.re[m] * cs - md.im[m] * sn);
+    }
+    const double d = fabs((double) out[row * Ng + z] - e);
+    emax = d > emax ? d : emax;
+    rmax = fabs(e) > rmax ? fabs(e) : rmax;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double a = __shfl_xor(emax, o, 64), b = __shfl_xor(rmax, o, 64);
+    emax = a > emax ? a : emax;
+    rmax = b > rmax ? b : rmax;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&res[0], (unsigned long long) __double_as_longlong(emax));
+    atomicMax(&res[1], (unsigned long long) __double_as_longlong(rmax));
+  }
+}
+
+hipError_t launch_plane_wave_check(const real *out, int Ng, const FftModes &md,
+    unsigned long long *res, hipStream_t st) {
+  hipLaunchKernelGGL(k_plane_wave_check, dim3((unsigned) ((size_t) Ng * Ng)), dim3(256), 0, st,
+      out, Ng, md, res);
+  return hipGetLastError();
+}

@@ -24,6 +24,15 @@ hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     real *ref_copy, hipStream_t st);
 hipError_t launch_unpack_r2c(cplx *buf, cplx *ny2, int Ng, hipStream_t st);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
+// Plane-wave check of a c2r output (see k_plane_wave_check).
+#define EZ_FFT_MAX_MODES 4
+struct FftModes {
+  int n;
+  int ijk[EZ_FFT_MAX_MODES][3];
+  double re[EZ_FFT_MAX_MODES], im[EZ_FFT_MAX_MODES], w[EZ_FFT_MAX_MODES];
+};
+hipError_t launch_plane_wave_check(const real *out, int Ng, const FftModes &md,
+    unsigned long long *res, hipStream_t st);
 hipError_t launch_scale(real *p, real s, size_t n, hipStream_t st);
 
 // cic.hip
@@ -121,10 +130,37 @@ hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
     const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st);
 // mtw (MT19937, else nullptr): stream words of draws 8N + 6 q, pair q.
+// Tracers t < natt (attached to particles) keep the velocity already in
+// vx/vy/vz; the others take the CIC gather; all get the sigma_v scatter.
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
-    const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    const real4 *vel, size_t ntot, size_t natt, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
     real *vz, hipStream_t st);
+// tracers attached to particles (pop_tracer.c:517-582, 804-879)
+struct AttGeom {
+  int Ng;
+  real igs, gs;
+  size_t N;
+  FastDiv fNg;
+};
+hipError_t launch_att_count(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const int16_t *nt, uint32_t *np, uint32_t *ovf, hipStream_t st);
+// blockoff null: per-block contested-cell counts; else write cid / cinfo.
+hipError_t launch_att_cells(const int16_t *nt, uint32_t *np, size_t N, uint32_t *blockcnt,
+    const uint64_t *blockoff, uint32_t *cinfo, hipStream_t st);
+hipError_t launch_att_classify(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const int16_t *nt, const uint32_t *np, uint8_t *cls,
+    uint32_t *blockcnt, hipStream_t st);
+hipError_t launch_att_list(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const uint32_t *np, const uint8_t *cls, const uint64_t *blockoff,
+    uint32_t *plist, hipStream_t st);
+// blockoff null: per-block selected counts; else write the attached tracers.
+hipError_t launch_att_select(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, double vfac, const uint8_t *cls, const uint64_t *contoff,
+    const uint8_t *sel, uint32_t *blockcnt, const uint64_t *blockoff, real *const cat[6],
+    hipStream_t st);
+hipError_t launch_att_rest(const int16_t *nt, const uint32_t *np, size_t N, int16_t *rest,
+    uint32_t *blockcnt, hipStream_t st);
 hipError_t launch_uniform(MrgState seed, uint64_t start, size_t n,
     const uint32_t *jtab, double *out, hipStream_t st);
 hipError_t launch_uniform_mt(const uint32_t *w, size_t n, int pos, double *out,

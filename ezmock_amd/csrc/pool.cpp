@@ -3,14 +3,19 @@
 // The reference produces one mock per process call (run_mock.c, or the
 // Python loop of example/fit/fit_EZmock.ipynb over seeds and bias
 // parameters).  A pool runs many such requests on one device at once:
-//   * nslot worker threads, each owning engine instances (EZMOCK) with their
-//     own HIP stream, so the kernels and host round trips of several mocks
-//     overlap on the GPU;
+//   * nslot worker threads; each request runs on an engine instance
+//     (EZMOCK, with its own HIP stream), so the kernels and host round trips
+//     of several mocks overlap on the GPU;
+//   * instances stay resident after their request, keyed by grid size and
+//     box: a later request of the same shape reuses an idle one (fields and
+//     scratch buffers included), so steady traffic allocates and frees no
+//     device memory -- hipFree synchronises the whole device, and churning
+//     GB-sized allocations between grid sizes stalled every worker;
 //   * a shared FIFO of requests; an idle worker takes the first queued
-//     request whose device footprint fits the memory budget next to what
-//     the other workers hold, preferring a request of the grid its cached
-//     instance already has (fields and scratch buffers are kept resident
-//     between requests of one grid size);
+//     request that has an idle instance of its shape, or whose device
+//     footprint fits the memory budget next to every resident instance,
+//     after evicting least recently used idle instances of other shapes if
+//     that makes room;
 //   * every request is computed by the same calls as a stand-alone mock
 //     (EZmock_set_cosmology, EZmock_setup_linear_pk,
 //     EZmock_create_dens_field, EZmock_ext_populate_device and optionally
@@ -47,11 +52,13 @@ struct Done {
   EZMOCK_RESULT res;
 };
 
-struct Slot {
-  EZMOCK *ez = nullptr;
+struct Inst {
+  EZMOCK *ez = nullptr;         // null while being created
   int Ng = 0;
   double Lbox = 0;
-  size_t held = 0;              // bytes accounted to this slot's instance
+  size_t bytes = 0;             // footprint accounted to the budget
+  bool busy = false;
+  double last_used = 0;
 };
 
 }  // namespace
@@ -66,8 +73,8 @@ struct EZMOCK_POOL {
   std::condition_variable cv_work, cv_done;
   std::deque<Job> queue;
   std::map<long, Done> done;
-  std::vector<Slot> slots;
-  size_t held = 0;              // sum of the slots' held bytes
+  std::vector<Inst *> insts;    // resident instances
+  size_t held = 0;              // sum of their bytes
   long next_id = 1;
   bool stop = false;
   std::vector<std::thread> threads;
@@ -86,24 +93,24 @@ size_t EZmock_ext_footprint(const int Ngrid, const int with_powspec) {
 
 namespace {
 
-int run_one(EZMOCK_POOL *p, Slot &sl, const Job &job, EZMOCK_RESULT &r) {
+int init_inst(EZMOCK_POOL *p, Inst &in, uint64_t seed) {
+  int err = 0;
+  in.ez = EZmock_init(in.Lbox, in.Ng, 0, seed, 0, &err);
+  if (!in.ez) return err ? err : EZMOCK_ERR_UNKNOWN;
+  EZmock_set_cosmology(in.ez, 0, 0, true, p->z, p->z_pk, p->Om, p->Onu, p->w, &err);
+  EZmock_setup_linear_pk(in.ez, p->k.data(), (int) p->k.size(), p->P.data(),
+      nullptr, 0, false, &err);
+  return err;
+}
+
+int run_one(EZMOCK_POOL *p, Inst &in, const Job &job, EZMOCK_RESULT &r) {
   const EZMOCK_REQUEST &q = job.req;
   int err = 0;
-  if (!sl.ez || sl.Ng != q.Ngrid || sl.Lbox != q.Lbox) {
-    if (sl.ez) EZmock_destroy(sl.ez);
-    sl.ez = EZmock_init(q.Lbox, q.Ngrid, 0, q.seed, 0, &err);
-    if (!sl.ez) return err ? err : EZMOCK_ERR_UNKNOWN;
-    sl.Ng = q.Ngrid;
-    sl.Lbox = q.Lbox;
-    EZmock_set_cosmology(sl.ez, 0, 0, true, p->z, p->z_pk, p->Om, p->Onu, p->w, &err);
-    EZmock_setup_linear_pk(sl.ez, p->k.data(), (int) p->k.size(), p->P.data(),
-        nullptr, 0, false, &err);
-    if (err) return err;
-  }
-  EZmock_ext_set_seed(sl.ez, q.seed, &err);
-  EZmock_create_dens_field(sl.ez, nullptr, false, nullptr, false, false, &err);
+  if (!in.ez && (err = init_inst(p, in, q.seed))) return err;
+  EZmock_ext_set_seed(in.ez, q.seed, &err);
+  EZmock_create_dens_field(in.ez, nullptr, false, nullptr, false, false, &err);
   size_t ntr = 0;
-  EZmock_ext_populate_device(sl.ez, q.params, q.nexp, false, &ntr, &err);
+  EZmock_ext_populate_device(in.ez, q.params, q.nexp, false, &ntr, &err);
   if (err) return err;
   r.ntracer = ntr;
   if (q.pk_nbin > 0) {
@@ -113,7 +120,7 @@ int run_one(EZMOCK_POOL *p, Slot &sl, const Job &job, EZMOCK_RESULT &r) {
     r.P2 = (double *) malloc(sizeof(double) * nb);
     r.nmodes = (double *) malloc(sizeof(double) * nb);
     if (!r.k || !r.P0 || !r.P2 || !r.nmodes) return EZMOCK_ERR_MEMORY;
-    EZmock_ext_powspec(sl.ez, q.rsd_fac, true, q.pk_dk, nb, r.k, r.P0, r.P2,
+    EZmock_ext_powspec(in.ez, q.rsd_fac, true, q.pk_dk, nb, r.k, r.P0, r.P2,
         r.nmodes, &err);
     if (err) return err;
   }
@@ -122,64 +129,120 @@ int run_one(EZMOCK_POOL *p, Slot &sl, const Job &job, EZMOCK_RESULT &r) {
       r.cat[i] = (real *) malloc(sizeof(real) * ntr);
       if (!r.cat[i]) return EZMOCK_ERR_MEMORY;
     }
-    EZmock_ext_copy_tracers(sl.ez, r.cat[0], r.cat[1], r.cat[2], r.cat[3],
+    EZmock_ext_copy_tracers(in.ez, r.cat[0], r.cat[1], r.cat[2], r.cat[3],
         r.cat[4], r.cat[5], ntr, &err);
   }
-  else EZmock_ext_sync(sl.ez, &err);
+  else EZmock_ext_sync(in.ez, &err);
   return err;
 }
 
-// Index of the queued job this slot may take, or -1.
-long pick(EZMOCK_POOL *p, const Slot &sl) {
-  long first_fit = -1;
+// What a worker will do next: run queued job `job` on resident instance
+// `inst` (or on a new one, inst = nullptr, after evicting `evict`).
+struct Plan {
+  long job = -1;
+  Inst *inst = nullptr;
+  std::vector<Inst *> evict;
+};
+
+// The first queued job that can run now (caller holds the lock).
+bool pick(EZMOCK_POOL *p, Plan &pl) {
   for (size_t i = 0; i < p->queue.size(); i++) {
     const EZMOCK_REQUEST &q = p->queue[i].req;
+    for (Inst *in : p->insts)
+      if (!in->busy && in->Ng == q.Ngrid && in->Lbox == q.Lbox) {
+        pl.job = (long) i;
+        pl.inst = in;
+        pl.evict.clear();
+        return true;
+      }
     const size_t need = EZmock_ext_footprint(q.Ngrid, q.pk_nbin > 0);
-    const bool same = sl.ez && sl.Ng == q.Ngrid && sl.Lbox == q.Lbox;
-    const size_t after = p->held - sl.held + std::max(need, same ? sl.held : need);
-    if (after > p->budget && p->held > sl.held) continue;   // wait for memory
-    if (same) return (long) i;
-    if (first_fit < 0) first_fit = (long) i;
+    if (p->held + need <= p->budget) {
+      pl.job = (long) i;
+      pl.inst = nullptr;
+      pl.evict.clear();
+      return true;
+    }
+    // idle instances of other shapes, least recently used first
+    std::vector<Inst *> idle;
+    for (Inst *in : p->insts)
+      if (!in->busy) idle.push_back(in);
+    std::sort(idle.begin(), idle.end(),
+        [](const Inst *a, const Inst *b) { return a->last_used < b->last_used; });
+    size_t freed = 0;
+    std::vector<Inst *> ev;
+    for (Inst *in : idle) {
+      if (p->held - freed + need <= p->budget) break;
+      ev.push_back(in);
+      freed += in->bytes;
+    }
+    if (p->held - freed + need <= p->budget) {
+      pl.job = (long) i;
+      pl.inst = nullptr;
+      pl.evict = ev;
+      return true;
+    }
   }
-  return first_fit;
+  return false;
 }
 
-void worker(EZMOCK_POOL *p, int si) {
+void drop(EZMOCK_POOL *p, Inst *in) {        // caller holds the lock
+  p->insts.erase(std::find(p->insts.begin(), p->insts.end(), in));
+  p->held -= in->bytes;
+}
+
+void worker(EZMOCK_POOL *p) {
   std::unique_lock<std::mutex> lk(p->mu);
-  Slot &sl = p->slots[si];
   for (;;) {
-    long idx = -1;
-    p->cv_work.wait(lk, [&] {
-      if (p->stop) return true;
-      idx = pick(p, sl);
-      return idx >= 0;
-    });
+    Plan pl;
+    p->cv_work.wait(lk, [&] { return p->stop || pick(p, pl); });
     if (p->stop) break;
-    Job job = p->queue[idx];
-    p->queue.erase(p->queue.begin() + idx);
+    Job job = p->queue[pl.job];
+    p->queue.erase(p->queue.begin() + pl.job);
     const size_t need = EZmock_ext_footprint(job.req.Ngrid, job.req.pk_nbin > 0);
-    const bool same = sl.ez && sl.Ng == job.req.Ngrid && sl.Lbox == job.req.Lbox;
-    const size_t hold = same ? std::max(need, sl.held) : need;
-    p->held = p->held - sl.held + hold;
-    sl.held = hold;
+    for (Inst *in : pl.evict) drop(p, in);
+    Inst *in = pl.inst;
+    if (!in) {
+      in = new Inst;
+      in->Ng = job.req.Ngrid;
+      in->Lbox = job.req.Lbox;
+      in->bytes = need;
+      p->insts.push_back(in);
+      p->held += need;
+    }
+    else if (need > in->bytes) {             // now with the estimator grids
+      p->held += need - in->bytes;
+      in->bytes = need;
+    }
+    in->busy = true;
     lk.unlock();
+    for (Inst *ev : pl.evict) {
+      EZmock_destroy(ev->ez);
+      delete ev;
+    }
     EZMOCK_RESULT r;
     memset(&r, 0, sizeof(r));
     r.id = job.id;
     r.t_submit = job.t_submit;
     r.t_start = now_s();
-    r.err = run_one(p, sl, job, r);
+    r.err = run_one(p, *in, job, r);
     r.t_end = now_s();
+    EZMOCK *bad = nullptr;
     lk.lock();
-    if (r.err && sl.ez) {            // an instance with a sticky error is dropped
-      EZmock_destroy(sl.ez);
-      sl.ez = nullptr;
-      p->held -= sl.held;
-      sl.held = 0;
+    in->busy = false;
+    in->last_used = r.t_end;
+    if (r.err) {                     // an instance with a sticky error is dropped
+      bad = in->ez;
+      drop(p, in);
+      delete in;
     }
     p->done[job.id] = Done{r};
     p->cv_done.notify_all();
-    p->cv_work.notify_all();         // memory accounting may admit others
+    p->cv_work.notify_all();         // idle instances and memory may admit others
+    if (bad) {
+      lk.unlock();
+      EZmock_destroy(bad);
+      lk.lock();
+    }
   }
 }
 
@@ -198,8 +261,7 @@ EZMOCK_POOL *EZmock_ext_pool_init(const int nslot, const size_t mem_budget,
   p->P.assign(Pk, Pk + nk);
   p->z = z; p->z_pk = z_pk; p->Om = Omega_m; p->Onu = Omega_nu; p->w = w;
   p->budget = mem_budget;
-  p->slots.resize(nslot);
-  for (int i = 0; i < nslot; i++) p->threads.emplace_back(worker, p, i);
+  for (int i = 0; i < nslot; i++) p->threads.emplace_back(worker, p);
   return p;
 }
 
@@ -246,8 +308,10 @@ void EZmock_ext_pool_destroy(EZMOCK_POOL *p) {
     p->cv_work.notify_all();
   }
   for (auto &t : p->threads) t.join();
-  for (auto &sl : p->slots)
-    if (sl.ez) EZmock_destroy(sl.ez);
+  for (Inst *in : p->insts) {
+    if (in->ez) EZmock_destroy(in->ez);
+    delete in;
+  }
   for (auto &d : p->done) EZmock_ext_result_free(&d.second.res);
   delete p;
 }

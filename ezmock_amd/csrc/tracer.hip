@@ -759,8 +759,8 @@ __device__ __forceinline__ void corners_vel(const Corners &k, double vfac, real 
 
 __global__ void __launch_bounds__(TPB)
 k_velocities(const real *__restrict__ x, const real *__restrict__ y,
-    const real *__restrict__ z, const real4 *__restrict__ vel, size_t ntot, int Ng, real igs, double vfac,
-    double sigma, uint64_t base_draw, MrgState seed,
+    const real *__restrict__ z, const real4 *__restrict__ vel, size_t ntot, size_t natt,
+    int Ng, real igs, double vfac, double sigma, uint64_t base_draw, MrgState seed,
     const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
     real *__restrict__ vx,
     real *__restrict__ vy, real *__restrict__ vz) {
@@ -794,7 +794,8 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     corners_load(vel, Ng, igs, x[a], y[a], z[a], ka);
     corners_load(vel, Ng, igs, x[bb], y[bb], z[bb], kb);
     real va[3], vb3[3];
-    corners_vel(ka, vfac, va);
+    if (a < natt) { va[0] = vx[a]; va[1] = vy[a]; va[2] = vz[a]; }
+    else corners_vel(ka, vfac, va);
     double g1, g2;
     gauss_pair(g, sigma, g1, g2);
     va[0] = (real) ((double) va[0] + g1);
@@ -803,7 +804,8 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     va[2] = (real) ((double) va[2] + g1);
     vx[a] = va[0]; vy[a] = va[1]; vz[a] = va[2];
     if (hb) {
-      corners_vel(kb, vfac, vb3);
+      if (b < natt) { vb3[0] = vx[b]; vb3[1] = vy[b]; vb3[2] = vz[b]; }
+      else corners_vel(kb, vfac, vb3);
       vb3[0] = (real) ((double) vb3[0] + g2);
       gauss_pair(g, sigma, g1, g2);
       vb3[1] = (real) ((double) vb3[1] + g1);
@@ -814,7 +816,7 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
 }
 
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
-    const real4 *vel, size_t ntot, int Ng, real igs, double vfac, double sigma_v,
+    const real4 *vel, size_t ntot, size_t natt, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
     real *vz, hipStream_t st) {
   const size_t N = (size_t) Ng * Ng * Ng;
@@ -823,8 +825,262 @@ hipError_t launch_velocities(const real *x, const real *y, const real *z,
   MrgState s8 = seed;                       // draw 8N
   mrg_jump(s8, (uint64_t) 8 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_velocities, dim3(ez_blocks(npair, TPB * PPT)), dim3(TPB),
-      0, st, x, y, z, vel, ntot, Ng, igs, vfac, sigma_v,
+      0, st, x, y, z, vel, ntot, natt, Ng, igs, vfac, sigma_v,
       (uint64_t) 0, s8, jtab, mtw, vx, vy, vz);
+  return hipGetLastError();
+}
+
+// ------------------------------------------- tracers attached to particles
+// pop_tracer.c:517-582 (NGP particle counts of the tracer cells) and
+// :804-879 (the selection).  Particles are numbered in Lagrangian order
+// q = (i Ng + j) Ng + k; each block takes CPB consecutive particles, CPT per
+// thread in a contiguous run, so block-scan ranks follow Lagrangian order.
+
+// Wrapped grid coordinate of particle q and its nearest grid point cell
+// (pop_tracer.c:536-557), in the reference's operation order.
+__device__ __forceinline__ size_t att_ngp(const real *__restrict__ p0,
+    const real *__restrict__ p1, const real *__restrict__ p2, size_t q,
+    const AttGeom &g, real pc[3]) {
+  uint32_t k, j;
+  const uint32_t ij = fastdiv_qr((uint32_t) q, g.fNg, k);
+  const uint32_t i = fastdiv_qr(ij, g.fNg, j);
+  const uint32_t lat[3] = {i, j, k};
+  const real ps[3] = {p0[q], p1[q], p2[q]};
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    real x = ps[d] * g.igs + (real) (int) lat[d];
+    if (x < 0) x += (real) g.Ng;
+    else if (x >= (real) g.Ng) x -= (real) g.Ng;
+    pc[d] = x;
+    c[d] = (int) ((double) x + 0.5);
+    if (c[d] >= g.Ng) c[d] = 0;
+    // displacements beyond a box length or not finite (undefined in the
+    // reference): keep the index inside the grid
+    if ((unsigned) c[d] >= (unsigned) g.Ng) c[d] = ((c[d] % g.Ng) + g.Ng) % g.Ng;
+  }
+  return ((size_t) c[0] * g.Ng + c[1]) * g.Ng + c[2];
+}
+
+// Particles per tracer cell (u32 counters, cleared by the caller); ovf is
+// set when a cell would exceed the reference's uint16 counter.
+__global__ void __launch_bounds__(TPB)
+k_att_count(const real *__restrict__ p0, const real *__restrict__ p1,
+    const real *__restrict__ p2, AttGeom g, const int16_t *__restrict__ nt,
+    uint32_t *__restrict__ np, uint32_t *__restrict__ ovf) {
+  const size_t q0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  for (int u = 0; u < CPT; u++) {
+    const size_t q = q0 + u;
+    if (q >= g.N) break;
+    real pc[3];
+    const size_t c = att_ngp(p0, p1, p2, q, g, pc);
+    if (nt[c] == 0) continue;
+    if (atomicAdd(&np[c], 1u) >= 65535u) *ovf = 1u;
+  }
+}
+
+// Contested cells (more particles than tracers: the selection draws) per
+// block of CPB cells; WRITE: give each its rank cid, record (nt, np) in
+// cinfo[cid] and mark np[c] = 0x80000000 | cid.
+template <int WRITE>
+__global__ void __launch_bounds__(TPB)
+k_att_cells(const int16_t *__restrict__ nt, uint32_t *__restrict__ np, size_t N,
+    uint32_t *__restrict__ blockcnt, const uint64_t *__restrict__ blockoff,
+    uint32_t *__restrict__ cinfo) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t c0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t n = 0;
+  for (int u = 0; u < CPT; u++) {
+    const size_t c = c0 + u;
+    if (c < N && nt[c] > 0 && np[c] > (uint32_t) nt[c]) n++;
+  }
+  uint32_t tot;
+  const uint32_t excl = block_excl_scan<uint32_t>(n, tot, ws);
+  if (!WRITE) {
+    if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+    return;
+  }
+  uint32_t r = (uint32_t) blockoff[blockIdx.x] + excl;
+  for (int u = 0; u < CPT; u++) {
+    const size_t c = c0 + u;
+    if (c < N && nt[c] > 0 && np[c] > (uint32_t) nt[c]) {
+      cinfo[r] = (uint32_t) (uint16_t) nt[c] | (np[c] << 16);
+      np[c] = 0x80000000u | r;
+      r++;
+    }
+  }
+}
+
+// Class of each particle: 0 its cell hosts no tracer, 1 attached without a
+// draw (the cell has no more particles than tracers), 2 contested; per
+// block the number of contested particles.
+__global__ void __launch_bounds__(TPB)
+k_att_classify(const real *__restrict__ p0, const real *__restrict__ p1,
+    const real *__restrict__ p2, AttGeom g, const int16_t *__restrict__ nt,
+    const uint32_t *__restrict__ np, uint8_t *__restrict__ cls,
+    uint32_t *__restrict__ blockcnt) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t q0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t n = 0;
+  for (int u = 0; u < CPT; u++) {
+    const size_t q = q0 + u;
+    if (q >= g.N) break;
+    real pc[3];
+    const size_t c = att_ngp(p0, p1, p2, q, g, pc);
+    const uint8_t k = nt[c] == 0 ? 0 : ((np[c] & 0x80000000u) ? 2 : 1);
+    cls[q] = k;
+    n += k == 2;
+  }
+  uint32_t tot;
+  (void) block_excl_scan<uint32_t>(n, tot, ws);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+// The contested particles in Lagrangian order: plist[rank] = their cell's
+// cid.
+__global__ void __launch_bounds__(TPB)
+k_att_list(const real *__restrict__ p0, const real *__restrict__ p1,
+    const real *__restrict__ p2, AttGeom g, const uint32_t *__restrict__ np,
+    const uint8_t *__restrict__ cls, const uint64_t *__restrict__ blockoff,
+    uint32_t *__restrict__ plist) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t q0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t n = 0;
+  for (int u = 0; u < CPT; u++)
+    if (q0 + u < g.N && cls[q0 + u] == 2) n++;
+  uint32_t tot;
+  uint64_t r = blockoff[blockIdx.x] + block_excl_scan<uint32_t>(n, tot, ws);
+  for (int u = 0; u < CPT; u++) {
+    const size_t q = q0 + u;
+    if (q >= g.N || cls[q] != 2) continue;
+    real pc[3];
+    const size_t c = att_ngp(p0, p1, p2, q, g, pc);
+    plist[r++] = np[c] & 0x7fffffffu;
+  }
+}
+
+// Selected particles (class 1, or class 2 with its selection flag) per
+// block; WRITE: the attached tracers at their rank: position = wrapped grid
+// coordinate x gs, velocity = displacement x vfac (pop_tracer.c:856-863).
+template <int WRITE>
+__global__ void __launch_bounds__(TPB)
+k_att_select(const real *__restrict__ p0, const real *__restrict__ p1,
+    const real *__restrict__ p2, AttGeom g, double vfac, const uint8_t *__restrict__ cls,
+    const uint64_t *__restrict__ contoff, const uint8_t *__restrict__ sel,
+    uint32_t *__restrict__ blockcnt, const uint64_t *__restrict__ blockoff,
+    real *__restrict__ x, real *__restrict__ y, real *__restrict__ z,
+    real *__restrict__ vx, real *__restrict__ vy, real *__restrict__ vz) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t q0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t nc = 0;
+  for (int u = 0; u < CPT; u++)
+    if (q0 + u < g.N && cls[q0 + u] == 2) nc++;
+  uint32_t tot;
+  uint64_t rc = contoff[blockIdx.x] + block_excl_scan<uint32_t>(nc, tot, ws);
+  uint32_t take = 0;                    // bit u: particle q0 + u is selected
+  for (int u = 0; u < CPT; u++) {
+    const size_t q = q0 + u;
+    if (q >= g.N) break;
+    const uint8_t k = cls[q];
+    if (k == 1 || (k == 2 && sel[rc])) take |= 1u << u;
+    if (k == 2) rc++;
+  }
+  uint64_t r = block_excl_scan<uint32_t>((uint32_t) __popc(take), tot, ws);
+  if (!WRITE) {
+    if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+    return;
+  }
+  r += blockoff[blockIdx.x];
+  for (int u = 0; u < CPT; u++) {
+    if (!((take >> u) & 1u)) continue;
+    const size_t q = q0 + u;
+    real pc[3];
+    (void) att_ngp(p0, p1, p2, q, g, pc);
+    x[r] = pc[0] * g.gs;
+    y[r] = pc[1] * g.gs;
+    z[r] = pc[2] * g.gs;
+    vx[r] = (real) ((double) p0[q] * vfac);
+    vy[r] = (real) ((double) p1[q] * vfac);
+    vz[r] = (real) ((double) p2[q] * vfac);
+    r++;
+  }
+}
+
+// Tracers per cell left for uniform positions: max(nt - np, 0), i.e. nt - np
+// in uncontested tracer cells and 0 in contested ones; per-block sums.
+__global__ void __launch_bounds__(TPB)
+k_att_rest(const int16_t *__restrict__ nt, const uint32_t *__restrict__ np, size_t N,
+    int16_t *__restrict__ rest, uint32_t *__restrict__ blockcnt) {
+  __shared__ uint32_t ws[TPB / 64];
+  const size_t c0 = (size_t) blockIdx.x * CPB + (size_t) threadIdx.x * CPT;
+  uint32_t n = 0;
+  for (int u = 0; u < CPT; u++) {
+    const size_t c = c0 + u;
+    if (c >= N) break;
+    const int t = nt[c];
+    const uint32_t p = np[c];
+    const int r = (t > 0 && !(p & 0x80000000u)) ? t - (int) p : 0;
+    rest[c] = (int16_t) r;
+    n += (uint32_t) r;
+  }
+  uint32_t tot;
+  (void) block_excl_scan<uint32_t>(n, tot, ws);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot;
+}
+
+hipError_t launch_att_count(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const int16_t *nt, uint32_t *np, uint32_t *ovf, hipStream_t st) {
+  hipLaunchKernelGGL(k_att_count, dim3(ez_blocks(g.N, CPB)), dim3(TPB), 0, st, p0, p1, p2,
+      g, nt, np, ovf);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_cells(const int16_t *nt, uint32_t *np, size_t N, uint32_t *blockcnt,
+    const uint64_t *blockoff, uint32_t *cinfo, hipStream_t st) {
+  if (!blockoff)
+    hipLaunchKernelGGL(k_att_cells<0>, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st, nt, np, N,
+        blockcnt, blockoff, cinfo);
+  else
+    hipLaunchKernelGGL(k_att_cells<1>, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st, nt, np, N,
+        blockcnt, blockoff, cinfo);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_classify(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const int16_t *nt, const uint32_t *np, uint8_t *cls,
+    uint32_t *blockcnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_att_classify, dim3(ez_blocks(g.N, CPB)), dim3(TPB), 0, st, p0, p1, p2,
+      g, nt, np, cls, blockcnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_list(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, const uint32_t *np, const uint8_t *cls, const uint64_t *blockoff,
+    uint32_t *plist, hipStream_t st) {
+  hipLaunchKernelGGL(k_att_list, dim3(ez_blocks(g.N, CPB)), dim3(TPB), 0, st, p0, p1, p2,
+      g, np, cls, blockoff, plist);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_select(const real *p0, const real *p1, const real *p2,
+    const AttGeom &g, double vfac, const uint8_t *cls, const uint64_t *contoff,
+    const uint8_t *sel, uint32_t *blockcnt, const uint64_t *blockoff, real *const cat[6],
+    hipStream_t st) {
+  if (!blockoff)
+    hipLaunchKernelGGL(k_att_select<0>, dim3(ez_blocks(g.N, CPB)), dim3(TPB), 0, st, p0, p1,
+        p2, g, vfac, cls, contoff, sel, blockcnt, blockoff, cat[0], cat[1], cat[2], cat[3],
+        cat[4], cat[5]);
+  else
+    hipLaunchKernelGGL(k_att_select<1>, dim3(ez_blocks(g.N, CPB)), dim3(TPB), 0, st, p0, p1,
+        p2, g, vfac, cls, contoff, sel, blockcnt, blockoff, cat[0], cat[1], cat[2], cat[3],
+        cat[4], cat[5]);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_rest(const int16_t *nt, const uint32_t *np, size_t N, int16_t *rest,
+    uint32_t *blockcnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_att_rest, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st, nt, np, N, rest,
+      blockcnt);
   return hipGetLastError();
 }
 

@@ -106,12 +106,15 @@ int EZmock_ext_powspec(EZMOCK *ez, const double rsd_fac, const bool interlace,
 int EZmock_ext_set_seed(EZMOCK *ez, const uint64_t seed, int *err);
 
 /* ---- request pool: continuous batching of mocks on one GPU ----
-   nslot worker threads, each with its own engine instance and HIP stream,
-   take queued requests as slots and device memory (mem_budget bytes, by the
-   estimate of EZmock_ext_footprint) free up; requests of one grid size
-   reuse the resident buffers of a worker.  Every request runs the same
-   calls as a stand-alone mock with the pool's cosmology (set_cosmology with
-   eval = true) and linear P(k) (no BAO modification, linear spline). */
+   nslot worker threads run queued requests concurrently, each on an engine
+   instance with its own HIP stream.  Instances stay resident, keyed by grid
+   size and box, and an idle one serves the next request of its shape; a
+   request without one starts when its device footprint (the estimate of
+   EZmock_ext_footprint) fits mem_budget next to every resident instance,
+   after evicting least recently used idle instances of other shapes if that
+   makes room.  Every request runs the same calls as a stand-alone mock with
+   the pool's cosmology (set_cosmology with eval = true) and linear P(k) (no
+   BAO modification, linear spline). */
 typedef struct EZMOCK_POOL EZMOCK_POOL;
 
 typedef struct {
@@ -180,6 +183,18 @@ int EZmock_ext_fft_r2c(const int Ng, const real *in, real *out, int *err);
 int EZmock_ext_pdf_walk(const real *rhot, const size_t num,
     const size_t ncell, const size_t *pdf, const int nmax, int16_t *nt,
     size_t *pdf_out, size_t *ncell_out, size_t *ntot_out, int *err);
+
+/* Plane-wave check of the c2r transforms at any supported grid size, on
+   the device: a spectrum of nmodes (<= 4) modes (ijk[3m..3m+2] = i, j, kk
+   with 0 < kk < Ng/2, amplitude amp[2m] + i amp[2m+1]) goes through the
+   generic pass sequence (path 0: one output, the modes as given) or the
+   production first pass of the three displacement components (path 1:
+   three outputs, the modes times n_x, n_y, n_z), and each output is
+   compared with sum_m 2 w_m Re(A_m exp(2 pi i (i x + j y + kk z) / Ng)).
+   maxerr[c] / maxref[c]: largest |output - closed form| and |closed form|
+   of output c. */
+int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const int *ijk,
+    const double *amp, double *maxerr, double *maxref, int *err);
 
 #ifdef __cplusplus
 }

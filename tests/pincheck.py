@@ -246,3 +246,79 @@ def identical_before_first_difference(nt_a, cat_a, nt_b, cat_b, vtol):
         d = np.abs(np.asarray(a, np.float64)[:n] - np.asarray(b, np.float64)[:n])
         assert n == 0 or d.max() <= vtol * vmax
     return n
+
+
+def att_expected(ez, Ng, L, seed, prec, rng_kind=0):
+    """Attached tracers of a populate call with att_part (pop_tracer.c:
+    517-582, 804-879), replayed independently: NGP cells of the displaced
+    particles in `real` arithmetic (wrapped once), particles per tracer
+    cell, and Algorithm S (Bentley, Programming Pearls 12.2) over the
+    particles in Lagrangian order with draws from stream position 4N, one
+    per particle of a cell that still has fewer tracers than particles.
+    Returns (indices of the attached particles in catalogue order, NGP
+    coordinate array p (N x 3, real), tracers left per cell)."""
+    rt = rtype(prec)
+    N = Ng ** 3
+    igs = rt(Ng / L)
+    q = np.arange(Ng)
+    lat = [a.ravel() for a in np.meshgrid(q, q, q, indexing="ij")]
+    p = np.empty((N, 3), rt)
+    c = np.empty((N, 3), np.int64)
+    for d, f in enumerate(("psi_x", "psi_y", "psi_z")):
+        x = (ez.field(f).astype(rt) * igs + lat[d].astype(rt)).astype(rt)
+        x = np.where(x < 0, x + rt(Ng), np.where(x >= rt(Ng), x - rt(Ng), x)).astype(rt)
+        p[:, d] = x
+        cd = (x.astype(np.float64) + 0.5).astype(np.int64)
+        c[:, d] = np.where(cd >= Ng, 0, cd)
+    cell = (c[:, 0] * Ng + c[:, 1]) * Ng + c[:, 2]
+    nt = ez.field("nt").astype(np.int64)
+    host = nt[cell] > 0
+    npart = np.bincount(cell[host], minlength=N)
+    t, n = nt.copy(), npart.copy()
+    ncontested = int(np.sum(host & (nt[cell] < npart[cell])))
+    u = po.Rng(rng_kind, seed, 4 * N, prec="f64").double(ncontested)
+    taken, d = [], 0
+    for i in np.flatnonzero(host):
+        ci = cell[i]
+        if t[ci] == 0:
+            continue
+        if t[ci] < n[ci]:
+            ran = u[d]
+            d += 1
+            if ran * n[ci] >= t[ci]:
+                n[ci] -= 1
+                continue
+        t[ci] -= 1
+        n[ci] -= 1
+        taken.append(i)
+    rest = np.maximum(nt - npart, 0)
+    assert np.array_equal(t, rest)
+    return np.array(taken, np.int64), p, rest
+
+
+def check_att(ez, cat, Ng, L, seed, prec, vfac, rng_kind=0):
+    """The catalogue of a populate call with att_part and sigma_v = 0:
+    attached tracers first (positions p gs, velocities psi vfac of the
+    replayed selection), then the others at uniform positions of draws
+    5N + 3t' (t' counting only them) in the cells left with tracers."""
+    rt = rtype(prec)
+    taken, p, rest = att_expected(ez, Ng, L, seed, prec, rng_kind)
+    iv = len(taken)
+    assert len(cat[0]) == iv + rest.sum()
+    gs = rt(L / Ng)
+    for d in range(3):
+        assert np.array_equal(np.asarray(cat[d])[:iv], (p[taken, d] * gs).astype(rt)), d
+    for d, f in enumerate(("psi_x", "psi_y", "psi_z")):
+        ref = (ez.field(f)[taken].astype(np.float64) * vfac).astype(rt)
+        assert np.array_equal(np.asarray(cat[3 + d])[:iv], ref), d
+    # the rest: host cells in cell order, u from 5N + 3t'
+    N = Ng ** 3
+    cells = np.repeat(np.arange(N), rest)
+    idx = np.unravel_index(cells, (Ng, Ng, Ng))
+    u = po.Rng(rng_kind, seed, 5 * N, prec="f64").double(3 * len(cells)).reshape(-1, 3)
+    tol = 1e-12 if prec == "f64" else 2.5e-7 * Ng
+    for col, i in enumerate(idx):
+        rec = np.asarray(cat[col], np.float64)[iv:] / float(gs) - i + 0.5
+        dd = np.mod(rec - u[:, col] + Ng / 2, Ng) - Ng / 2
+        assert len(dd) == 0 or np.abs(dd).max() <= tol, col
+    return iv

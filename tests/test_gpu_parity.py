@@ -225,12 +225,6 @@ def test_unsupported_configurations_fail_loudly(gpu_available, pk_table):
     g.setup_linear_pk(k, P)
     with pytest.raises(RuntimeError, match="invalid grid size"):
         g.create_dens_field_from_ic()
-    g = EZmock(Lbox=320., Ngrid=32, rng=0)
-    g.eval_growth_params(0.5, 0.31)
-    g.setup_linear_pk(k, P)
-    g.create_dens_field_from_ic()
-    with pytest.raises(RuntimeError):
-        g.populate_tracer(0.5, 10., 0.3, 150., 600, att_part=True)
 
 
 def test_config1_256_fp64_against_oracle(gpu_available, pk_table):

@@ -113,3 +113,34 @@ def test_identical_fraction_counts_shifted_tracers():
     moved = tuple(np.concatenate([a[:half], a[half:] + 1e-3]) for a in cat)
     f = pc.identical_tracer_fraction(nt, moved, nt, cat, 0)
     assert abs(f - half / len(cat[0])) < 1e-12
+
+
+@pytest.mark.parametrize("prec,rng", [("f64", 0), ("f32", 0), ("f64", 1)])
+def test_att_part_selection_replayed(prec, rng):
+    """att_part (pop_tracer.c:517-582, 804-879): the oracle's catalogue
+    against an independent replay of the NGP counts and the Bentley
+    selection on stream 4N, and the other tracers' positions on 5N + 3t'."""
+    from ezmock_amd import inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = po.EZmockOracle(Lbox=96., Ngrid=24, rng=rng, seed=5, nthread=1, prec=prec)
+    ez.eval_growth_params(0.0, 0.31)
+    ez.setup_linear_pk(k, P)
+    ez.create_dens_field_from_ic()
+    cat = ez.populate_tracer(0.4, 8., 0.3, 0., 800, att_part=True)
+    iv = pc.check_att(ez, cat, 24, 96., 5, prec, ez.info()["vfac"], rng)
+    assert iv == ez.info()["iv"]
+    assert 0 < iv < len(cat[0])                       # both kinds present
+    taken, _, rest = pc.att_expected(ez, 24, 96., 5, prec, rng)
+    assert rest.sum() > 0
+
+
+def test_att_check_rejects_wrong_stream():
+    from ezmock_amd import inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = po.EZmockOracle(Lbox=96., Ngrid=24, rng=0, seed=5, nthread=1, prec="f64")
+    ez.eval_growth_params(0.0, 0.31)
+    ez.setup_linear_pk(k, P)
+    ez.create_dens_field_from_ic()
+    cat = ez.populate_tracer(0.4, 8., 0.3, 0., 800, att_part=True)
+    with pytest.raises(AssertionError):
+        pc.check_att(ez, cat, 24, 96., 6, "f64", ez.info()["vfac"])   # another seed's draws
