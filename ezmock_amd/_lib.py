@@ -29,6 +29,7 @@ EXT_SYMBOLS = [
     "EZmock_ext_powspec", "EZmock_ext_set_seed", "EZmock_ext_footprint",
     "EZmock_ext_pool_init", "EZmock_ext_pool_submit", "EZmock_ext_pool_wait",
     "EZmock_ext_result_free", "EZmock_ext_pool_destroy", "EZmock_ext_fft_check",
+    "EZmock_ext_noalloc_scratch_bytes", "EZmock_ext_pdf_walk_noalloc",
 ]
 
 

@@ -118,6 +118,9 @@ hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
     unsigned long long *flag, hipStream_t st);
 hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
     size_t cap, unsigned long long *counter, hipStream_t st);
+// 64-bin log2 histogram of nonzero tracer densities, added to hist.
+hipError_t launch_hist_rhot(const real *rhot, size_t N, unsigned long long *hist,
+    hipStream_t st);
 hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
     uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st);
 size_t sort_hist_words(size_t n);

@@ -285,6 +285,49 @@ hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
   return hipGetLastError();
 }
 
+// Log2 histogram of a tracer-density array (pop_tracer.c:265-268 with
+// ilogb_max 5; zero densities are not counted): 64 u64 counters, which the
+// caller clears.  Lanes of a wave that share the first active lane's bin add
+// with one LDS atomic (densities concentrate in a few bins).
+__global__ void __launch_bounds__(TPB)
+k_hist_rhot(const real *__restrict__ rhot, size_t N, unsigned long long *__restrict__ hist) {
+  __shared__ uint32_t h[EZ_NUM_LOGB_BIN];
+  if (threadIdx.x < EZ_NUM_LOGB_BIN) h[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  real v[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    const real t = rhot[i < N ? i : N - 1];
+    v[q] = (i < N) ? t : (real) 0;
+  }
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const bool on = v[q] != 0;
+    const int d = on ? hist_bin(v[q], 5) : 0;
+    uint64_t act = __ballot(on);
+    while (act) {
+      const int leader = __ffsll((unsigned long long) act) - 1;
+      const int ld = __shfl(d, leader, 64);
+      const uint64_t peers = __ballot(on && d == ld) & act;
+      if (lane == leader) atomicAdd(&h[ld], (uint32_t) __popcll(peers));
+      act &= ~peers;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < EZ_NUM_LOGB_BIN && h[threadIdx.x])
+    atomicAdd(&hist[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+}
+
+hipError_t launch_hist_rhot(const real *rhot, size_t N, unsigned long long *hist,
+    hipStream_t st) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hist_rhot, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st, rhot, N, hist);
+  return hipGetLastError();
+}
+
 // -------------------------------------------------------------- compact
 // Keys: bitwise NOT of the bits of a positive real, so that ascending key
 // order is descending density.

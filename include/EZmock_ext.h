@@ -196,6 +196,27 @@ int EZmock_ext_pdf_walk(const real *rhot, const size_t num,
 int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const int *ijk,
     const double *amp, double *maxerr, double *maxref, int *err);
 
+/* ---- no-allocation entry point (no counterpart in the reference) ----
+   For callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops): the PDF walk of EZmock_ext_pdf_walk (the
+   selection of pop_tracer.c:359-514) on device-resident arrays, calling no
+   hipMalloc/hipFree.  rhot_dev: `num` tracer densities, nt_dev: `num` int16
+   tracers per cell (output), both device memory of the current device;
+   pdf[nmax], pdf_out[nmax], *ncell_out and *ntot_out are host memory.
+   stream: a hipStream_t (NULL: the null stream); the call synchronises it.
+   Every temporary lives in one static device buffer per GPU
+   (EZmock_ext_noalloc_scratch_bytes(), held until the process ends), so at
+   most EZMOCK_NOALLOC_MAX_CELLS cells (else EZMOCK_ERR_ARG_NGRID) and
+   EZMOCK_NOALLOC_MAX_CAND candidate cells, those in the log2 density bins
+   down to the threshold bin (else EZMOCK_ERR_ARG_HIEXP).  The buffer keeps
+   the last call's contents; calls that overlap on one GPU race on it. */
+#define EZMOCK_NOALLOC_MAX_CELLS ((size_t) 1 << 31)
+#define EZMOCK_NOALLOC_MAX_CAND ((size_t) 1 << 22)
+size_t EZmock_ext_noalloc_scratch_bytes(void);
+int EZmock_ext_pdf_walk_noalloc(const real *rhot_dev, const size_t num,
+    const size_t ncell, const size_t *pdf, const int nmax, int16_t *nt_dev,
+    size_t *pdf_out, size_t *ncell_out, size_t *ntot_out, void *stream, int *err);
+
 #ifdef __cplusplus
 }
 #endif
