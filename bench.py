@@ -348,6 +348,10 @@ def main():
             d2, n2, _, _ = run_gpu(1024, 2500., 42, params, density, "f32", 2, 1, profile=False)
             extra["ms_per_mock_1024_f32"] = round(d2 * 1e3 / 2, 2)
             log(f"bench: 1024^3 f32 {extra['ms_per_mock_1024_f32']} ms/mock")
+            # per-kernel times of one more mock with the timers on
+            _, _, kt2, _ = run_gpu(1024, 2500., 42, params, density, "f32", 1, 1, profile=True)
+            extra["kernel_ms_1024_f32"] = {n: round(t, 3) for n, (t, c) in kt2.items()}
+            log(f"bench: 1024^3 kernels {extra['kernel_ms_1024_f32']}")
         except Exception as e:          # report, do not hide
             extra["ms_per_mock_1024_f32"] = f"failed: {e}"
         try:

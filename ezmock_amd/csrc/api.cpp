@@ -319,9 +319,13 @@ void cic_pending(Mesh *m, int kind, int Ng, real igs, real *rho, bool with_mom) 
 // List entries per tile with fixed ranges: 4 per cell of the tile, about
 // three times the mean (1.35 per cell); dense tiles rarely exceed twice it
 // at cell sizes of a few Mpc/h.  After an overflow the ranges are resized
-// from the exact counts, up to CIC_FIXED_MAX_PER_CELL.
+// from the exact counts: up to CIC_FIXED_MAX_PER_CELL always, and up to
+// CIC_FIXED_MEM_PER_CELL where the device keeps a quarter of its memory
+// free after the larger lists (small cells cluster more: 1024^3 in a
+// 2500 Mpc/h box needs about 8).
 #define CIC_FIXED_PER_CELL 4
 #define CIC_FIXED_MAX_PER_CELL 6
+#define CIC_FIXED_MEM_PER_CELL 16
 
 // CIC by particle binning: scatter into fixed ranges per tile and tiles,
 // checked for overflow after the next synchronisation (no count pass, no
@@ -470,7 +474,21 @@ int cic_finish(Mesh *m) {
       uint32_t mx = 0;
       for (uint32_t c : cnt) mx = c > mx ? c : mx;
       const uint32_t cap = ((mx + mx / 4) + 63) / 64 * 64;
-      if (cap <= (uint32_t) (CIC_FIXED_MAX_PER_CELL * g.TX * g.TY * g.TZ)) {
+      const size_t tc = (size_t) g.TX * g.TY * g.TZ;
+      bool ok = cap <= CIC_FIXED_MAX_PER_CELL * tc;
+      if (!ok && cap <= CIC_FIXED_MEM_PER_CELL * tc) {
+        // the exact list is freed before the fixed one is allocated
+        size_t fr = 0, tot = 0;
+        const size_t need = nk * (size_t) cap * sizeof(uint32_t);
+        const size_t have = m->cic_list_cap * sizeof(uint32_t);
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && need <= have + fr && 
+            have + fr - need >= tot / 4)
+          ok = true;
+        else (void) hipGetLastError();
+      }
+      if (ok && m->cic_list_limit && nk * (size_t) cap * sizeof(uint32_t) > m->cic_list_limit)
+        ok = false;
+      if (ok) {
         m->cic_fcap = cap;
         m->cic_fixed_ok = true;
       }

@@ -80,10 +80,22 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   int k = (int) (idx0 - row * (size_t) Ngk);
   int i = (int) (row / (size_t) Ng);
   int j = (int) (row - (size_t) i * Ng);
-  MrgState g = gp.seed_state;
   const uint64_t mult = gp.fixamp ? 1 : 2;
   uint64_t d = (idx0 == 0 ? 0 : (idx0 - 1)) * mult;   // next draw (MT19937 words)
-  if (!MT && idx0 < gp.Nh) mrg_jump(g, d, jtab);
+  // MRG32k3a: one jump from the seed per workgroup (to the draw of its
+  // first cell), then a short jump per thread (< 2 base-256 digits)
+  __shared__ MrgState gbase;
+  const uint64_t dblk = (blk0 == 0 ? 0 : (blk0 - 1)) * mult;
+  if (!MT) {
+    if (threadIdx.x == 0) {
+      MrgState b = gp.seed_state;
+      mrg_jump(b, dblk, jtab);
+      gbase = b;
+    }
+    __syncthreads();
+  }
+  MrgState g = gbase;
+  if (!MT && idx0 < gp.Nh) mrg_jump(g, d - dblk, jtab);
   for (int c = 0; c < CH; c++) {
     const size_t idx = idx0 + c;
     const int e = threadIdx.x * CH + c;
@@ -122,8 +134,10 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
           P *= gp.Pfac;
           if (gp.iphase) ph += EZ_PI;
           P /= (ksq * gp.kfac);
-          re = (real) (-P * sin(ph));
-          im = (real) (P * cos(ph));
+          double sn, cs;
+          sincos(ph, &sn, &cs);           // one argument reduction for both
+          re = (real) (-P * sn);
+          im = (real) (P * cs);
           // perturb.c:341-359: self-conjugate modes keep only the imaginary
           // part (the reference writes (+P sin, P cos) twice, then zeroes re).
           if (mirror && i == ni && j == nj) re = 0;

@@ -97,6 +97,8 @@ int init_inst(EZMOCK_POOL *p, Inst &in, uint64_t seed) {
   int err = 0;
   in.ez = EZmock_init(in.Lbox, in.Ng, 0, seed, 0, &err);
   if (!in.ez) return err ? err : EZMOCK_ERR_UNKNOWN;
+  // keep the CIC lists within the footprint's 24 bytes per cell
+  EZmock_ext_set_option(in.ez, 5, (long) (24 * (size_t) in.Ng * in.Ng * in.Ng));
   EZmock_set_cosmology(in.ez, 0, 0, true, p->z, p->z_pk, p->Om, p->Onu, p->w, &err);
   EZmock_setup_linear_pk(in.ez, p->k.data(), (int) p->k.size(), p->P.data(),
       nullptr, 0, false, &err);

@@ -171,8 +171,11 @@ __device__ __forceinline__ void gauss_pair(Draws &s, double sigma,
     double &g1, double &g2) {
   const double r = -2 * log(s.next_pos());
   const double th = s.next() * 2 * EZ_PI;
-  g1 = sigma * sqrt(r) * cos(th);
-  g2 = sigma * sqrt(r) * sin(th);
+  // one argument reduction for both (the values of separate sin and cos)
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  g1 = sigma * sqrt(r) * cs;
+  g2 = sigma * sqrt(r) * sn;
 }
 
 // LDS tile with one pad word per 16 so that thread-sequential access

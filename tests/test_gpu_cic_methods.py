@@ -203,16 +203,20 @@ def compressed(Ng, a, b):
     return [(i / a - i).ravel(), (j / b - j).ravel(), np.zeros(Ng ** 3)]
 
 
-def test_fixed_list_resize_and_second_overflow(gpu_available, pk_table):
+@pytest.mark.parametrize("limit", [False, True])
+def test_fixed_list_resize_and_second_overflow(gpu_available, pk_table, limit):
     """One instance, four fields (32^3, FP64, 8x8x16 tiles of 1024 cells):
       A  fullest tile 4.03 entries per cell: overflows the default ranges
          (4 per cell), is repainted with exact ranges (count pass), and the
          ranges of later fields become 1.25 x 4131 -> 5184 entries;
       B  fullest tile 4.18 per cell: fits the resized ranges (no count pass,
          although it would have overflowed the default ones);
-      C  fullest tile 5.31 per cell: overflows the resized ranges; 1.25 x
-         that exceeds 6 per cell, so the instance keeps exact ranges;
-      D  the field of B again: exact ranges (count pass).
+      C  fullest tile 5.84 per cell: overflows the resized ranges; 1.25 x
+         that is 7.3 per cell, above 6: with the device's free memory the
+         ranges grow to it (up to 16 per cell), while with option 5 limiting
+         the lists to 24 bytes per cell the instance keeps exact ranges;
+      D  the field of B again: fixed ranges, or exact ranges (count pass)
+         under the limit.
     Every density matches the oracle's, and the catalogues of B and D
     match the oracle's tracer for tracer."""
     Ng, L = 32, 32.
@@ -223,8 +227,10 @@ def test_fixed_list_resize_and_second_overflow(gpu_available, pk_table):
     assert 4096 < mx["A"] <= 4915                    # overflows; resize <= 6 / cell
     cap = (mx["A"] + mx["A"] // 4 + 63) // 64 * 64
     assert 4096 < mx["B"] <= cap < mx["C"]
-    assert mx["C"] + mx["C"] // 4 > 6 * 1024
+    assert 6 * 1024 < mx["C"] + mx["C"] // 4 <= 16 * 1024
     g = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=3, precision="f64")
+    if limit:
+        assert g.set_option(5, 24 * Ng ** 3) == 0
     g.profile(True)
     g.eval_growth_params(0.5, 0.31)
     count_pass = {}
@@ -243,7 +249,7 @@ def test_fixed_list_resize_and_second_overflow(gpu_available, pk_table):
             assert pc.identical_tracer_fraction(g.field("nt"), cg, o.field("nt"), co,
                                                 1e-9) >= 0.9999, name
         o.close()
-    assert count_pass == {"A": True, "B": False, "C": True, "D": True}
+    assert count_pass == {"A": True, "B": False, "C": True, "D": limit}
 
 
 def test_fixed_list_memory_limit_takes_exact_ranges(gpu_available, pk_table):
