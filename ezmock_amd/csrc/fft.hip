@@ -236,8 +236,13 @@ k_fft_strided(C *__restrict__ buf, const C *__restrict__ rhok,
     const C *__restrict__ rhok_ny, const C *__restrict__ tw, int Ng, int Ngh,
     size_t stride, size_t outer_stride, int comp, int axis) {
   __shared__ C lds[L * TK];
-  const int kk0 = blockIdx.x * TK;
-  const int o = blockIdx.y;
+  // 1-D grid of (Ngh / TK) x Ng tiles, in XCD-aware order: the workgroups
+  // of one XCD take consecutive tiles, so column tiles that share cache
+  // lines (TK * sizeof(C) < 128 B) run together behind one L2
+  const unsigned nx = (unsigned) (Ngh / TK);
+  const unsigned t = xcd_item(blockIdx.x, gridDim.x);
+  const int kk0 = (int) (t % nx) * TK;
+  const int o = (int) (t / nx);
   const size_t base = (size_t) o * outer_stride + kk0;
   constexpr int E = L * TK;
   if (MODE == 0) {
@@ -326,8 +331,11 @@ k_fft_first3(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
   __shared__ C lds[L * TK];
   constexpr int E = L * TK;
   constexpr int IT = (E + NT - 1) / NT;
-  const int kk0 = blockIdx.x * TK;
-  const int o = blockIdx.y;                 // j
+  // 1-D grid in XCD-aware order (see k_fft_strided)
+  const unsigned nx = (unsigned) (Ngh / TK);
+  const unsigned t = xcd_item(blockIdx.x, gridDim.x);
+  const int kk0 = (int) (t % nx) * TK;
+  const int o = (int) (t / nx);             // j
   const size_t stride = (size_t) Ng * Ngh;
   const size_t base = (size_t) o * Ngh + kk0;
   const int ko = (o <= Ngh) ? o : o - Ng;
@@ -522,7 +530,7 @@ static hipError_t launch_strided_tk(cplx *buf, const cplx *rhok,
   const size_t plane = (size_t) Ng * Ngh;
   const size_t stride = (axis == 0) ? plane : (size_t) Ngh;
   const size_t outer = (axis == 0) ? (size_t) Ngh : plane;
-  dim3 grid(Ngh / TK, Ng);
+  dim3 grid((unsigned) ((Ngh / TK) * (size_t) Ng));
   if (mode == 0)
     hipLaunchKernelGGL((k_fft_strided<cplx, L, TK, NT, SIGN, 0>), grid, dim3(NT), 0, st,
         buf, rhok, rhok_ny, tw, Ng, Ngh, stride, outer, comp, axis);
@@ -549,8 +557,8 @@ static hipError_t launch_first3_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *rho
     const cplx *slot0, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
   constexpr int E = L * T;
   constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
-  hipLaunchKernelGGL((k_fft_first3<cplx, L, T, NT>), dim3(Ngh / T, Ng), dim3(NT), 0, st,
-      o0, o1, o2, rhok, slot0, tw, Ng, Ngh);
+  hipLaunchKernelGGL((k_fft_first3<cplx, L, T, NT>), dim3((unsigned) ((Ngh / T) * (size_t) Ng)),
+      dim3(NT), 0, st, o0, o1, o2, rhok, slot0, tw, Ng, Ngh);
   return hipGetLastError();
 }
 
