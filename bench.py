@@ -365,6 +365,12 @@ def main():
             log(f"bench: 1536^3 {extra['large_1536_f32']}")
         except Exception as e:
             extra["large_1536_f32"] = f"failed: {e}"
+    try:                                # the autotuner's picks (tk 0 = default)
+        from ezmock_amd.EZmock import fft_schedule
+        extra["fft_schedules"] = {str(n): fft_schedule(n, args.precision)
+                                  for n in (256, 384, 512, 1024, 1536)}
+    except Exception as e:
+        extra["fft_schedules"] = f"failed: {e}"
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: timing the CPU oracle sample ...")

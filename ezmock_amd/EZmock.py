@@ -359,6 +359,22 @@ def fft_r2c(x, precision="f32"):
     return out[..., 0] + 1j * out[..., 1]
 
 
+def fft_schedule(Ng, precision="f32"):
+    """FFT schedule of grid size Ng in this process (EZmock_ext_fft_schedule):
+    dict(tk_j, tk_i, split, tuned)."""
+    L = _lib.load(precision)
+    out = np.zeros(4, dtype=np.int32)
+    if L.EZmock_ext_fft_schedule(int(Ng), out.ctypes.data):
+        raise RuntimeError(f"no FFT schedule for grid size {Ng}")
+    return dict(tk_j=int(out[0]), tk_i=int(out[1]), split=int(out[2]), tuned=bool(out[3]))
+
+
+def fft_set_schedule(Ng, tk_j, tk_i, split, precision="f32"):
+    L = _lib.load(precision)
+    if L.EZmock_ext_fft_set_schedule(int(Ng), int(tk_j), int(tk_i), int(split)):
+        raise ValueError("invalid FFT schedule")
+
+
 def fft_check(Ng, path, modes, amps, precision="f32"):
     """Plane waves through the GPU c2r passes, checked on the device
     (EZmock_ext_fft_check): path 0 = generic passes (one output), path 1 =

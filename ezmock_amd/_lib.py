@@ -29,6 +29,7 @@ EXT_SYMBOLS = [
     "EZmock_ext_powspec", "EZmock_ext_set_seed", "EZmock_ext_footprint",
     "EZmock_ext_pool_init", "EZmock_ext_pool_submit", "EZmock_ext_pool_wait",
     "EZmock_ext_result_free", "EZmock_ext_pool_destroy", "EZmock_ext_fft_check",
+    "EZmock_ext_fft_schedule", "EZmock_ext_fft_set_schedule",
     "EZmock_ext_noalloc_scratch_bytes", "EZmock_ext_pdf_walk_noalloc",
 ]
 
@@ -93,6 +94,10 @@ def load(precision="f32"):
     L.EZmock_ext_uniform.argtypes = [i32, u64, u64, sz, b, vp, pint]
     L.EZmock_ext_fft_c2r.restype = i32
     L.EZmock_ext_fft_c2r.argtypes = [i32, vp, vp, pint]
+    L.EZmock_ext_fft_schedule.restype = i32
+    L.EZmock_ext_fft_schedule.argtypes = [i32, vp]
+    L.EZmock_ext_fft_set_schedule.restype = i32
+    L.EZmock_ext_fft_set_schedule.argtypes = [i32, i32, i32, i32]
     L.EZmock_ext_fft_check.restype = i32
     L.EZmock_ext_fft_check.argtypes = [i32, i32, i32, vp, vp, vp, vp, pint]
     L.EZmock_ext_fft_r2c.restype = i32

@@ -51,6 +51,7 @@ struct Mesh {
   hipStream_t sx[2] = {nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   bool fft_streams = true;
+  bool fft_tune = true;         // time the FFT schedule candidates on first use
   const uint32_t *jtab = nullptr;
   // fields (device)
   real *psi[3] = {nullptr, nullptr, nullptr};
@@ -503,7 +504,7 @@ int c2r_finish(Mesh *m, int Ng, real *out, hipStream_t st) {
   cplx *buf = (cplx *) out;
   {
     Scope s(m, "fft_pass_j", st);
-    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, st));
+    KCHK(fft_strided_pass(buf, nullptr, nullptr, m->twN, Ng, 1, +1, 0, 0, st, -1));
   }
   {
     Scope s(m, "fft_rows_c2r", st);
@@ -1339,10 +1340,20 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
         return (*err = EZMOCK_ERR_UNKNOWN);
       }
     }
+    // the first field of a grid size (>= 256) times the FFT schedule
+    // candidates on these buffers and keeps the fastest for the process
+    bool tuned = false;
+    (void) fft_schedule(Ng, &tuned);
+    if (m->fft_tune && Ng >= 256 && !tuned &&
+        fft_autotune(Ng, (cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2], rhok,
+            m->ny, m->slot0, m->twN, m->st, nullptr, nullptr) != hipSuccess) {
+      m->ny = ny_save;
+      return (*err = EZMOCK_ERR_UNKNOWN);
+    }
     {
       Scope s(m, "fft_pass_i");
       if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
-          rhok, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
+          rhok, m->ny, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
         m->ny = ny_save;
         return (*err = EZMOCK_ERR_UNKNOWN);
       }
@@ -1671,6 +1682,7 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
     m->cic_fixed_ok = true;
   }
   else if (option == 4) m->fft_streams = value != 0;
+  else if (option == 6) m->fft_tune = value != 0;
   else if (option == 5 && value >= 0) {
     m->cic_list_limit = (size_t) value;
     m->cic_fixed_ok = true;
@@ -1849,6 +1861,24 @@ static int fft_test(const int Ng, const real *in, real *out, int sign, int *err)
   return (*err = e);
 }
 
+int EZmock_ext_fft_schedule(const int Ng, int *out) {
+  if (!out) return EZMOCK_ERR_ARG_NULL;
+  if (!fft_supported(Ng)) return EZMOCK_ERR_ARG_NGRID;
+  bool tuned = false;
+  const FftSchedule sc = fft_schedule(Ng, &tuned);
+  out[0] = sc.tk_j; out[1] = sc.tk_i; out[2] = sc.split; out[3] = tuned ? 1 : 0;
+  return EZMOCK_SUCCESS;
+}
+
+int EZmock_ext_fft_set_schedule(const int Ng, const int tk_j, const int tk_i,
+    const int split) {
+  if (!fft_supported(Ng)) return EZMOCK_ERR_ARG_NGRID;
+  auto ok = [](int t) { return t == 0 || t == 4 || t == 8 || t == 16; };
+  if (!ok(tk_j) || !ok(tk_i) || (split != 0 && split != 1)) return EZMOCK_ERR_ARG_PARAMS;
+  fft_set_schedule(Ng, FftSchedule{tk_j, tk_i, split});
+  return EZMOCK_SUCCESS;
+}
+
 // Plane waves through the c2r passes at any supported size, checked on the
 // device against their closed form (no host copies of the fields).
 int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const int *ijk,
@@ -1903,14 +1933,14 @@ int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const i
     }
     if (!e && path == 0) {
       cplx *b = (cplx *) out[0];
-      if (fft_strided_pass(b, rhok, tmp.ny, tmp.twN, Ng, 0, +1, 1, 3, tmp.st) != hipSuccess ||
-          fft_strided_pass(b, nullptr, nullptr, tmp.twN, Ng, 1, +1, 0, 0, tmp.st) != hipSuccess ||
+      if (fft_strided_pass(b, rhok, tmp.ny, tmp.twN, Ng, 0, +1, 1, 3, tmp.st, -1) != hipSuccess ||
+          fft_strided_pass(b, nullptr, nullptr, tmp.twN, Ng, 1, +1, 0, 0, tmp.st, -1) != hipSuccess ||
           fft_row_pass(b, tmp.twN, tmp.twh, Ng, +1, tmp.st) != hipSuccess)
         e = EZMOCK_ERR_UNKNOWN;
     }
     else if (!e) {
       if (fft_slot0_pack(rhok, tmp.ny, tmp.slot0, Ng, tmp.st) != hipSuccess ||
-          fft_first_pass3((cplx *) out[0], (cplx *) out[1], (cplx *) out[2], rhok, tmp.slot0,
+          fft_first_pass3((cplx *) out[0], (cplx *) out[1], (cplx *) out[2], rhok, tmp.ny, tmp.slot0,
               tmp.twN, Ng, tmp.st) != hipSuccess)
         e = EZMOCK_ERR_UNKNOWN;
       for (int c = 0; c < 3 && !e; c++) e = c2r_finish(&tmp, Ng, out[c], tmp.st);

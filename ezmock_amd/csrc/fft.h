@@ -9,9 +9,11 @@ bool fft_supported(int Ng);
 // field buf[Ng][Ng][Ng/2].  sign +1: e^{+ikx}, -1: e^{-ikx}.  mode 1 reads
 // the generated spectrum (rhok, rhok_ny) instead of buf and applies the
 // component multiply `comp` (0 x, 1 y, 2 z) and kz = 0 / Nyquist packing.
+// tk: column tile width (4, 8, 16; 0 = built-in default; -1 = the grid's
+// schedule, see fft_schedule).
 hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
     const cplx *tw, int Ng, int axis, int sign, int mode, int comp,
-    hipStream_t st);
+    hipStream_t st, int tk = 0);
 
 // Row pass along kk: sign +1 = c2r (packed half spectrum -> Ng reals per
 // row), -1 = r2c.  twN: exp(+2 pi i m / Ng), twh: exp(+2 pi i m / (Ng/2)).
@@ -23,7 +25,24 @@ hipError_t fft_row_pass(cplx *buf, const cplx *twN, const cplx *twh, int Ng,
 hipError_t fft_slot0_pack(const cplx *rhok, const cplx *rhok_ny, cplx *slot0,
     int Ng, hipStream_t st);
 
+// Schedule of a grid size: tile widths of the strided j pass (tk_j) and of
+// the first pass (tk_i; 0 = defaults), and the first pass fused (split = 0:
+// the spectrum read once, three transforms per workgroup) or split into
+// three single-component strided passes (split = 1).
+struct FftSchedule { int tk_j, tk_i, split; };
+FftSchedule fft_schedule(int Ng, bool *tuned);
+void fft_set_schedule(int Ng, const FftSchedule &s);
+
 // First c2r pass (axis i) of the three displacement components rhok * n_a,
-// a = x, y, z, into out0..out2 (spectrum read once; kk = 0 from slot0).
+// a = x, y, z, into out0..out2 (kk = 0 from slot0, or from rhok / rhok_ny
+// when split), by `sched` or the grid's schedule (null).
 hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
-    const cplx *slot0, const cplx *tw, int Ng, hipStream_t st);
+    const cplx *rhok_ny, const cplx *slot0, const cplx *tw, int Ng, hipStream_t st,
+    const FftSchedule *sched = nullptr);
+
+// Times every candidate schedule of grid Ng on the given buffers (out0..2
+// overwritten, the spectrum read) and records the fastest for the grid;
+// ms_best[0], [1]: best times of the j pass and of the first pass.
+hipError_t fft_autotune(int Ng, cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *slot0, const cplx *tw, hipStream_t st,
+    FftSchedule *best, double *ms_best);

@@ -18,6 +18,10 @@
 //   row pass (c2r): TR contiguous rows; Hermitian pre-processing
 //     Z[m] = X[m] + conj(X[L/2-m]) + i W^m (X[m] - conj(X[L/2-m])), an L/2
 //     complex FFT, and the row of L reals is the complex result.
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "common.h"
 #include "fft.h"
 
@@ -540,8 +544,15 @@ static hipError_t launch_strided_tk(cplx *buf, const cplx *rhok,
   return hipGetLastError();
 }
 
+// Column tile widths of the strided passes: the default (StridedCfg), or a
+// candidate of the schedule (4, 8, 16 columns, where Ngh allows it and the
+// tile fits the 160 KB of LDS).
+template <int L, int TK> constexpr bool tk_fits() {
+  return L * TK * (int) sizeof(cplx) <= 160 * 1024;
+}
+
 template <int L, int SIGN>
-static hipError_t launch_strided(cplx *buf, const cplx *rhok,
+static hipError_t launch_strided_default(cplx *buf, const cplx *rhok,
     const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, int axis, int mode,
     int comp, hipStream_t st) {
   constexpr int T = StridedCfg<L>::TKMAX;
@@ -550,6 +561,27 @@ static hipError_t launch_strided(cplx *buf, const cplx *rhok,
   if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_strided_tk<L, 4, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
   if (Ngh % 2 == 0) return launch_strided_tk<L, 2, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
   return launch_strided_tk<L, 1, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+}
+
+// Sizes whose passes have schedule candidates (smaller grids: defaults).
+template <int L> constexpr bool tunable() { return L >= 256; }
+
+template <int L, int SIGN>
+static hipError_t launch_strided(cplx *buf, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *tw, int Ng, int Ngh, int axis, int mode,
+    int comp, int tk, hipStream_t st) {
+  if constexpr (tunable<L>()) {
+    if constexpr (tk_fits<L, 4>())
+      if (tk == 4 && Ngh % 4 == 0)
+        return launch_strided_tk<L, 4, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+    if constexpr (tk_fits<L, 8>())
+      if (tk == 8 && Ngh % 8 == 0)
+        return launch_strided_tk<L, 8, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+    if constexpr (tk_fits<L, 16>())
+      if (tk == 16 && Ngh % 16 == 0)
+        return launch_strided_tk<L, 16, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+  }
+  return launch_strided_default<L, SIGN>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
 }
 
 template <int L, int T>
@@ -572,7 +604,7 @@ template <int L> struct First3Cfg {
 };
 
 template <int L>
-static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
+static hipError_t launch_first3_default(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
     const cplx *slot0, const cplx *tw, int Ng, int Ngh, hipStream_t st) {
   constexpr int T = First3Cfg<L>::TKMAX;
   if (Ngh % T == 0) return launch_first3_tk<L, T>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
@@ -580,6 +612,23 @@ static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
   if constexpr (T >= 4) if (Ngh % 4 == 0) return launch_first3_tk<L, 4>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
   if (Ngh % 2 == 0) return launch_first3_tk<L, 2>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
   return launch_first3_tk<L, 1>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+}
+
+template <int L>
+static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
+    const cplx *slot0, const cplx *tw, int Ng, int Ngh, int tk, hipStream_t st) {
+  if constexpr (tunable<L>()) {
+    if constexpr (tk_fits<L, 4>())
+      if (tk == 4 && Ngh % 4 == 0)
+        return launch_first3_tk<L, 4>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+    if constexpr (tk_fits<L, 8>())
+      if (tk == 8 && Ngh % 8 == 0)
+        return launch_first3_tk<L, 8>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+    if constexpr (tk_fits<L, 16>())
+      if (tk == 16 && Ngh % 16 == 0)
+        return launch_first3_tk<L, 16>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+  }
+  return launch_first3_default<L>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
 }
 
 template <int Lh>
@@ -608,14 +657,40 @@ bool fft_supported(int Ng) {
   }
 }
 
+// ------------------------------------------------------------ schedules
+// Per grid size (and precision of this build): the column tile width of the
+// strided passes (axis j; and axis i where the first pass is split), the
+// tile width of the three-component first pass, and whether the first pass
+// runs as three single-component strided passes instead (each reads the
+// spectrum; smaller tiles in registers).  tk = 0 means the built-in
+// default.  Schedules come from fft_autotune (timing every candidate on
+// the caller's buffers) or fft_set_schedule.
+namespace {
+std::mutex g_sched_mu;
+std::map<int, FftSchedule> g_sched;
+}
+
+FftSchedule fft_schedule(int Ng, bool *tuned) {
+  std::lock_guard<std::mutex> lk(g_sched_mu);
+  auto it = g_sched.find(Ng);
+  if (tuned) *tuned = it != g_sched.end();
+  return it != g_sched.end() ? it->second : FftSchedule{0, 0, 0};
+}
+
+void fft_set_schedule(int Ng, const FftSchedule &s) {
+  std::lock_guard<std::mutex> lk(g_sched_mu);
+  g_sched[Ng] = s;
+}
+
 hipError_t fft_strided_pass(cplx *buf, const cplx *rhok, const cplx *rhok_ny,
     const cplx *tw, int Ng, int axis, int sign, int mode, int comp,
-    hipStream_t st) {
+    hipStream_t st, int tk) {
   const int Ngh = Ng / 2;
+  if (tk < 0) tk = fft_schedule(Ng, nullptr).tk_j;
   switch (Ng) {
 #define X(n) case n: return (sign > 0) \
-      ? launch_strided<n, +1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st) \
-      : launch_strided<n, -1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, st);
+      ? launch_strided<n, +1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, tk, st) \
+      : launch_strided<n, -1>(buf, rhok, rhok_ny, tw, Ng, Ngh, axis, mode, comp, tk, st);
     EZ_FFT_SIZES(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -642,12 +717,83 @@ hipError_t fft_slot0_pack(const cplx *rhok, const cplx *rhok_ny, cplx *slot0,
 }
 
 hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
-    const cplx *slot0, const cplx *tw, int Ng, hipStream_t st) {
+    const cplx *rhok_ny, const cplx *slot0, const cplx *tw, int Ng, hipStream_t st,
+    const FftSchedule *sched) {
   const int Ngh = Ng / 2;
+  const FftSchedule sc = sched ? *sched : fft_schedule(Ng, nullptr);
+  if (sc.split) {                       // three single-component passes
+    cplx *outs[3] = {out0, out1, out2};
+    for (int c = 0; c < 3; c++) {
+      const hipError_t e = fft_strided_pass(outs[c], rhok, rhok_ny, tw, Ng, 0, +1, 1, c, st,
+          sc.tk_i);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   switch (Ng) {
-#define X(n) case n: return launch_first3<n>(out0, out1, out2, rhok, slot0, tw, Ng, Ngh, st);
+#define X(n) case n: return launch_first3<n>(out0, out1, out2, rhok, slot0, tw, Ng, Ngh, sc.tk_i, st);
     EZ_FFT_SIZES(X)
 #undef X
     default: return hipErrorInvalidValue;
   }
+}
+
+// Times every candidate schedule of grid Ng on the caller's buffers (the
+// spectrum rhok / rhok_ny / slot0 is read; out0..out2 are overwritten) and
+// records the fastest: tile widths 4, 8 and 16 (where they fit), the first
+// pass fused or split.  Each candidate: one untimed run, then the best of
+// two timed runs.
+hipError_t fft_autotune(int Ng, cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
+    const cplx *rhok_ny, const cplx *slot0, const cplx *tw, hipStream_t st,
+    FftSchedule *best, double *ms_best) {
+  hipEvent_t a, b;
+  hipError_t e;
+  if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+  if ((e = hipEventCreate(&b)) != hipSuccess) { (void) hipEventDestroy(a); return e; }
+  auto time_it = [&](auto &&run, float &ms) -> hipError_t {
+    hipError_t r = run();
+    if (r != hipSuccess) return r;
+    ms = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      if ((r = hipEventRecord(a, st)) != hipSuccess || (r = run()) != hipSuccess ||
+          (r = hipEventRecord(b, st)) != hipSuccess || (r = hipEventSynchronize(b)) != hipSuccess)
+        return r;
+      float t = 0;
+      if ((r = hipEventElapsedTime(&t, a, b)) != hipSuccess) return r;
+      ms = t < ms ? t : ms;
+    }
+    return hipSuccess;
+  };
+  const int Ngh = Ng / 2;
+  const int tks[4] = {0, 4, 8, 16};
+  FftSchedule bs{0, 0, 0};
+  float best_j = 1e30f, best_i = 1e30f;
+  for (int t : tks) {
+    if (t && Ngh % t) continue;
+    if (t * Ng * (int) sizeof(cplx) > 160 * 1024) continue;
+    float ms;
+    e = time_it([&] { return fft_strided_pass(out0, nullptr, nullptr, tw, Ng, 1, +1, 0, 0, st, t); },
+        ms);
+    if (e != hipSuccess) break;
+    if (ms < best_j) { best_j = ms; bs.tk_j = t; }
+  }
+  for (int split = 0; split < 2 && e == hipSuccess; split++) {
+    for (int t : tks) {
+      if (t && Ngh % t) continue;
+      if (t * Ng * (int) sizeof(cplx) > 160 * 1024) continue;
+      const FftSchedule c{0, t, split};
+      float ms;
+      e = time_it([&] { return fft_first_pass3(out0, out1, out2, rhok, rhok_ny, slot0, tw, Ng,
+          st, &c); }, ms);
+      if (e != hipSuccess) break;
+      if (ms < best_i) { best_i = ms; bs.tk_i = t; bs.split = split; }
+    }
+  }
+  (void) hipEventDestroy(a);
+  (void) hipEventDestroy(b);
+  if (e != hipSuccess) return e;
+  fft_set_schedule(Ng, bs);
+  if (best) *best = bs;
+  if (ms_best) { ms_best[0] = best_j; ms_best[1] = best_i; }
+  return hipSuccess;
 }

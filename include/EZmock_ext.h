@@ -59,7 +59,9 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
                 per tile (default: 4 entries per cell; after an overflow
                 the field is repainted with exact ranges and the ranges of
                 later fields become 1.25 x the fullest tile, if that is at
-                most 6 per cell, else exact ranges), 1 = particle binning
+                most 6 per cell, or at most 16 per cell where a quarter of
+                the device memory stays free, else exact ranges), 1 =
+                particle binning
                 with exact ranges (count pass), 2 = tile scan of the
                 displaced lattice wherever the displacement range admits it
                 (no particle lists), else binning.  All give the same
@@ -69,7 +71,11 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
                 one stream while option 2 is on;
             5 = bytes of device memory the fixed list ranges may take
                 (0 = no limit, the default); above it, or when their
-                allocation fails, exact ranges are used. */
+                allocation fails, exact ranges are used;
+            6 = FFT autotuning (1, default): the first field of a grid size
+                >= 256 in the process times the schedule candidates
+                (EZmock_ext_fft_schedule) and keeps the fastest; 0 = use the
+                grid's current schedule. */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
@@ -183,6 +189,17 @@ int EZmock_ext_fft_r2c(const int Ng, const real *in, real *out, int *err);
 int EZmock_ext_pdf_walk(const real *rhot, const size_t num,
     const size_t ncell, const size_t *pdf, const int nmax, int16_t *nt,
     size_t *pdf_out, size_t *ncell_out, size_t *ntot_out, int *err);
+
+/* FFT schedule of grid size Ng in this process: out[0] = column tile width
+   of the strided (j) passes, out[1] = tile width of the first (i) pass
+   (0 = the built-in default, else 4, 8 or 16), out[2] = 1 if the first pass
+   runs as three single-component passes instead of the fused
+   three-component pass, out[3] = 1 if chosen by the autotuner (or set).
+   All schedules give bit-identical fields. */
+int EZmock_ext_fft_schedule(const int Ngrid, int *out);
+/* Sets the schedule of grid size Ng (tile widths 0, 4, 8, 16; split 0/1). */
+int EZmock_ext_fft_set_schedule(const int Ngrid, const int tk_j, const int tk_i,
+    const int split);
 
 /* Plane-wave check of the c2r transforms at any supported grid size, on
    the device: a spectrum of nmodes (<= 4) modes (ijk[3m..3m+2] = i, j, kk
