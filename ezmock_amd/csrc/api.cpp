@@ -489,6 +489,15 @@ int cic_finish(Mesh *m) {
       }
       if (ok && m->cic_list_limit && nk * (size_t) cap * sizeof(uint32_t) > m->cic_list_limit)
         ok = false;
+      // the larger list is allocated now, in the field that overflowed, so
+      // that the next fields run without an allocation (hipMalloc of tens
+      // of GB takes about a second)
+      if (ok && m->cic_list_cap < nk * (size_t) cap) {
+        dfree(&m->cic_list);
+        m->cic_list_cap = 0;
+        if (dalloc(&m->cic_list, nk * (size_t) cap)) ok = false;
+        else m->cic_list_cap = nk * (size_t) cap;
+      }
       if (ok) {
         m->cic_fcap = cap;
         m->cic_fixed_ok = true;
