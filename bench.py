@@ -144,8 +144,16 @@ def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
     dt = time.perf_counter() - t0
     pool.close()
     bad = [r["errmsg"] for r in res if r["err"]]
+    # one instance, one mock after another: its wall time, and the sum of
+    # its kernel times (the rate the pool would reach with no host gaps)
+    d1, _, _, _ = run_gpu(Ng, L, 1, par[0], 3e-4, "f32", 8, 2, profile=False)
+    _, _, kt1, _ = run_gpu(Ng, L, 1, par[0], 3e-4, "f32", 8, 1, profile=True)
+    kms = sum(t for t, c in kt1.values()) / 8
     return {"mocks": nmock, "grid": Ng, "slots": nslot, "seconds": round(dt, 3),
-            "mocks_per_s": round(nmock / dt, 2), "errors": bad[:3]}
+            "mocks_per_s": round(nmock / dt, 2), "errors": bad[:3],
+            "single_instance_ms_per_mock": round(d1 * 1e3 / 8, 3),
+            "kernel_ms_per_mock": round(kms, 3),
+            "kernel_bound_mocks_per_s": round(1e3 / kms, 1)}
 
 
 def run_serving(nreq=2000, rate=20.0, nslot=4, budget=200e9, seed=7):
@@ -290,10 +298,16 @@ def main():
     density = 2e-4
     s = 4 if args.precision == "f32" else 8
     log(f"bench: 512^3 {args.precision}, warmup {args.warmup}, steps {args.steps}")
-    dt, ntr, kt, lat = run_gpu(Ng, Lbox, seed, params, density, args.precision,
-                               args.steps, args.warmup)
+    # the timed run without the per-kernel timers (which put the three FFT
+    # components on one stream and add an event pair per kernel); then the
+    # same steps with the timers, for the kernel breakdown and the roofline
+    dt, ntr, _, lat = run_gpu(Ng, Lbox, seed, params, density, args.precision,
+                              args.steps, args.warmup, profile=False)
     ms = dt * 1e3 / args.steps
     log(f"bench: {ms:.2f} ms/mock, {ntr} tracers")
+    dtp, _, kt, _ = run_gpu(Ng, Lbox, seed, params, density, args.precision,
+                            args.steps, 1, profile=True)
+    log(f"bench: with per-kernel timers {dtp * 1e3 / args.steps:.2f} ms/mock")
     for name, (t, c) in sorted(kt.items(), key=lambda x: -x[1][0]):
         log(f"  {name:16s} {t / args.steps:9.3f} ms/mock  ({c} launches)")
 
