@@ -53,6 +53,7 @@ struct Mesh {
   bool fft_streams = true;
   bool fft_tune = true;         // time the FFT schedule candidates on first use
   const uint32_t *jtab = nullptr;
+  MrgState *gen_base = nullptr;  // k_gen_rhok workgroup states (MRG32k3a)
   // fields (device)
   real *psi[3] = {nullptr, nullptr, nullptr};
   real *rho = nullptr;          // also the spectrum buffer before the CIC
@@ -203,7 +204,7 @@ void free_tracer_scratch(Mesh *m) {
 
 void free_mesh_device(Mesh *m) {
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
-  dfree(&m->vel4);
+  dfree(&m->vel4); dfree(&m->gen_base);
   dfree(&m->rho); dfree(&m->ny); dfree(&m->slot0); dfree(&m->rhot); dfree(&m->nt);
   dfree(&m->ptab); dfree(&m->twN); dfree(&m->twh); dfree(&m->spec);
   dfree(&m->blockcnt); dfree(&m->blockoff); dfree(&m->scan_tmp);
@@ -1335,8 +1336,10 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     else {
       const uint32_t *mtw = nullptr;
       if ((e = mt_words(ez, m, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw))) return (*err = e);
+      if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return (*err = e);
       Scope s(m, "gen_spectrum");
-      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->st));
+      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
+          m->st));
     }
     // the generated Nyquist plane lives in ny[plane, 2 plane)
     cplx *ny_save = m->ny;

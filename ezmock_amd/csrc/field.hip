@@ -63,11 +63,27 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 #define GEN_TPB 256
 #define GEN_SKIP 0xffffu
 
+#define GEN_CHUNK (sizeof(real) == 4 ? 16 : 8)     // cells per thread
+
+// MRG32k3a state at the first draw of each k_gen_rhok workgroup, one thread
+// per workgroup, so that no workgroup waits on a serial jump of its own.
+__global__ void __launch_bounds__(256)
+k_gen_base(MrgState seed, uint64_t mult, size_t nblk, size_t cells_per_blk,
+    const uint32_t *__restrict__ jtab, MrgState *__restrict__ base) {
+  const size_t b = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const size_t blk0 = b * cells_per_blk;
+  MrgState g = seed;
+  mrg_jump(g, (blk0 == 0 ? 0 : blk0 - 1) * mult, jtab);
+  base[b] = g;
+}
+
 template <int CH, bool MT>
 __global__ void __launch_bounds__(GEN_TPB)
 k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
     real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
-    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw) {
+    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
+    const MrgState *__restrict__ bases) {
   constexpr int NB = GEN_TPB * CH;
   __shared__ cplx sv[NB + NB / CH];             // one pad slot per thread
   __shared__ uint16_t so[NB];
@@ -82,20 +98,14 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   int j = (int) (row - (size_t) i * Ng);
   const uint64_t mult = gp.fixamp ? 1 : 2;
   uint64_t d = (idx0 == 0 ? 0 : (idx0 - 1)) * mult;   // next draw (MT19937 words)
-  // MRG32k3a: one jump from the seed per workgroup (to the draw of its
-  // first cell), then a short jump per thread (< 2 base-256 digits)
-  __shared__ MrgState gbase;
+  // MRG32k3a: the workgroup's state at the draw of its first cell (from
+  // k_gen_base), then a short jump per thread (< 2 base-256 digits)
   const uint64_t dblk = (blk0 == 0 ? 0 : (blk0 - 1)) * mult;
+  MrgState g;
   if (!MT) {
-    if (threadIdx.x == 0) {
-      MrgState b = gp.seed_state;
-      mrg_jump(b, dblk, jtab);
-      gbase = b;
-    }
-    __syncthreads();
+    g = bases[blockIdx.x];
+    if (idx0 < gp.Nh) mrg_jump(g, d - dblk, jtab);
   }
-  MrgState g = gbase;
-  if (!MT && idx0 < gp.Nh) mrg_jump(g, d - dblk, jtab);
   for (int c = 0; c < CH; c++) {
     const size_t idx = idx0 + c;
     const int e = threadIdx.x * CH + c;
@@ -175,18 +185,29 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   }
 }
 
+size_t gen_base_count(size_t Nh) {
+  constexpr int CH = GEN_CHUNK;
+  return ez_blocks((Nh + CH - 1) / CH, GEN_TPB);
+}
+
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    const uint32_t *mtw, hipStream_t st) {
+    const uint32_t *mtw, MrgState *bases, hipStream_t st) {
   if (gp.Ng & 1) return hipErrorInvalidValue;
-  constexpr int CH = sizeof(real) == 4 ? 16 : 8;
+  constexpr int CH = GEN_CHUNK;
   const size_t nthr = (gp.Nh + CH - 1) / CH;
+  const size_t nblk = ez_blocks(nthr, GEN_TPB);
   if (mtw)
-    hipLaunchKernelGGL((k_gen_rhok<CH, true>), dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
-        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw);
-  else
-    hipLaunchKernelGGL((k_gen_rhok<CH, false>), dim3(ez_blocks(nthr, GEN_TPB)), dim3(GEN_TPB), 0,
-        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw);
+    hipLaunchKernelGGL((k_gen_rhok<CH, true>), dim3(nblk), dim3(GEN_TPB), 0,
+        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw, (const MrgState *) nullptr);
+  else {
+    if (!bases) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gen_base, dim3(ez_blocks(nblk, 256)), dim3(256), 0, st,
+        gp.seed_state, (uint64_t) (gp.fixamp ? 1 : 2), nblk, (size_t) GEN_TPB * CH, jtab,
+        bases);
+    hipLaunchKernelGGL((k_gen_rhok<CH, false>), dim3(nblk), dim3(GEN_TPB), 0,
+        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw, (const MrgState *) bases);
+  }
   return hipGetLastError();
 }
 

@@ -16,9 +16,12 @@ hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
     const double *Pt, const double *P2t, int n, int logpk, double kfac,
     double logkfac, hipStream_t st);
 // mtw: MT19937 stream words of draws 0 .. (mult (Nh - 1)) (nullptr: MRG32k3a)
+// bases: gen_base_count(gp.Nh) states of scratch (MRG32k3a; unused with
+// MT19937 words).
+size_t gen_base_count(size_t Nh);
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    const uint32_t *mtw, hipStream_t st);
+    const uint32_t *mtw, MrgState *bases, hipStream_t st);
 hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     const cplx *dk_ny, const GenParams &gp, const double *ptab,
     real *ref_copy, hipStream_t st);
