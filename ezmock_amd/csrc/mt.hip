@@ -75,22 +75,38 @@ __device__ void mt_jump_lds(const uint32_t *gl, uint32_t *S, uint32_t *acc,
         if (i < MT_N) acc[i] = keep[u];
       }
     }
-    // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words)
-#pragma unroll
-    for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-      const int i = tid + u * MT_TPB;
-      if (i >= MT_N) continue;
-      uint32_t t = 0;
+    // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words).
+    // Thread t < 156 owns words 4t .. 4t + 3: per coefficient word it reads
+    // the 35 window words those four need (nine aligned 16-byte LDS reads)
+    // and applies all 32 coefficients with masks, independent operations
+    // instead of a serial loop over the set bits.
+    if (tid < MT_N / 4) {
+      const int i0 = 4 * tid;
+      uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
       for (int wq = 0; wq < MT_B / 32; wq++) {
         const int gw = q * (MT_B / 32) + wq;
-        uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
-        while (bits) {
-          const int b = __ffs(bits) - 1;
-          bits &= bits - 1;
-          t ^= S[i + wq * 32 + b];
+        const uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
+        if (!bits) continue;            // the same for every thread
+        const uint4 *src = (const uint4 *) (S + i0 + wq * 32);   // up to S[847]
+        uint32_t w[36];
+#pragma unroll
+        for (int v = 0; v < 9; v++) {
+          const uint4 x = src[v];
+          w[4 * v] = x.x; w[4 * v + 1] = x.y; w[4 * v + 2] = x.z; w[4 * v + 3] = x.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+          const uint32_t m = 0u - ((bits >> r) & 1u);
+          t0 ^= w[r] & m;
+          t1 ^= w[r + 1] & m;
+          t2 ^= w[r + 2] & m;
+          t3 ^= w[r + 3] & m;
         }
       }
-      acc[i] ^= t;
+      acc[i0] ^= t0;
+      acc[i0 + 1] ^= t1;
+      acc[i0 + 2] ^= t2;
+      acc[i0 + 3] ^= t3;
     }
     __syncthreads();
   }
@@ -104,7 +120,7 @@ __global__ void __launch_bounds__(MT_TPB)
 k_mt_fan(const uint32_t *__restrict__ polys, const uint32_t *__restrict__ in,
     int fan, size_t nout, uint32_t *__restrict__ out) {
   __shared__ uint32_t gl[MT_JUMP_WORDS];
-  __shared__ uint32_t S[MT_N + MT_B];
+  __shared__ __align__(16) uint32_t S[MT_N + MT_B];
   __shared__ uint32_t acc[MT_N], nw[MT_B];
   const int tid = threadIdx.x;
   const size_t o = blockIdx.x;

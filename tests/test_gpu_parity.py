@@ -32,7 +32,8 @@ def test_uniform_draws_bit_exact(gpu_available):
 def test_uniform_draws_mt19937_bit_exact(gpu_available, seed):
     """MT19937 streams on the GPU (jump-ahead by x^e mod phi, segment
     generation) against the oracle's sequential generator: get_double and
-    get_double_pos, at offsets below and across one segment (2^18 words)."""
+    get_double_pos (two words each), at offsets below and across segments
+    of 2^18 words."""
     for start in (0, 1, 311, 312, 997, 131071, 131072, 300001):
         o = po.Rng(1, seed, start, prec="f64")
         ref = o.double(700)
@@ -42,13 +43,17 @@ def test_uniform_draws_mt19937_bit_exact(gpu_available, seed):
         assert np.array_equal(uniform(seed, start, 700, rng=1, pos=True), ref), (seed, start)
 
 
-def test_mt19937_long_stream_segments(gpu_available):
-    """A stream long enough for two levels of segment windows (> 16 x 2^18
-    words): spot checks against the oracle at the segment boundaries."""
-    n = 2_200_000                                   # 4.4 M words, 17 segments
+@pytest.mark.parametrize("n", [2_200_000, 34_000_000])
+def test_mt19937_long_stream_segments(gpu_available, n):
+    """Streams long enough for two and three levels of segment windows
+    (4.4 M words: 17 segments of 2^18 words; 68 M words: 260 segments):
+    spot checks against the oracle at segment and level boundaries."""
     g = uniform(7, 0, n, rng=1)
-    for d in (0, 131071, 131072, 131073, 2 * 131072, 16 * 131072 - 1, 16 * 131072,
-              16 * 131072 + 5, n - 3):
+    S = 131072                                      # doubles per segment
+    for d in (0, S - 1, S, S + 1, 2 * S, 16 * S - 1, 16 * S, 16 * S + 5, 67 * S + 3,
+              256 * S - 1, 256 * S, 259 * S + 7, n - 3):
+        if d >= n:
+            continue
         ref = po.Rng(1, 7, d, prec="f64").double(3)
         assert np.array_equal(g[d:d + 3], ref[:len(g[d:d + 3])]), d
 
