@@ -27,6 +27,7 @@
 #define MT_N 624
 #define MT_M 397
 #define MT_TPB 256
+#define MT_JTPB 640                    // jump workgroups: 4 x 156 XOR slots
 #define MT_B 224
 #define MT_SEG_WORDS (1ull << 18)
 #define MT_FAN 16
@@ -53,37 +54,40 @@ __device__ void mt_jump_lds(const uint32_t *gl, uint32_t *S, uint32_t *acc,
   for (int i = MT_JUMP_WORDS - 1; i >= 0; i--)
     if (gl[i]) { qtop = (i * 32) / MT_B; break; }
   // S[624 .. 624 + B - 2] = next words of the input window
-  for (int j = tid; j < MT_B - 1; j += MT_TPB)
+  for (int j = tid; j < MT_B - 1; j += MT_JTPB)
     S[MT_N + j] = mt_f(S[j], S[j + 1], S[j + MT_M]);
-  for (int i = tid; i < MT_N; i += MT_TPB) acc[i] = 0;
+  for (int i = tid; i < MT_N; i += MT_JTPB) acc[i] = 0;
   __syncthreads();
   for (int q = qtop; q >= 0; q--) {
     // acc <- A^B acc
     if (q != qtop) {
-      for (int j = tid; j < MT_B; j += MT_TPB) nw[j] = mt_f(acc[j], acc[j + 1], acc[j + MT_M]);
+      for (int j = tid; j < MT_B; j += MT_JTPB) nw[j] = mt_f(acc[j], acc[j + 1], acc[j + MT_M]);
       __syncthreads();
-      uint32_t keep[(MT_N + MT_TPB - 1) / MT_TPB];
+      uint32_t keep[(MT_N + MT_JTPB - 1) / MT_JTPB];
 #pragma unroll
-      for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-        const int i = tid + u * MT_TPB;
+      for (int u = 0; u < (MT_N + MT_JTPB - 1) / MT_JTPB; u++) {
+        const int i = tid + u * MT_JTPB;
         if (i < MT_N) keep[u] = (i + MT_B < MT_N) ? acc[i + MT_B] : nw[i + MT_B - MT_N];
       }
       __syncthreads();
 #pragma unroll
-      for (int u = 0; u < (MT_N + MT_TPB - 1) / MT_TPB; u++) {
-        const int i = tid + u * MT_TPB;
+      for (int u = 0; u < (MT_N + MT_JTPB - 1) / MT_JTPB; u++) {
+        const int i = tid + u * MT_JTPB;
         if (i < MT_N) acc[i] = keep[u];
       }
     }
     // acc ^= sum_r g_{qB + r} A^r W (B = 224 = 7 coefficient words).
-    // Thread t < 156 owns words 4t .. 4t + 3: per coefficient word it reads
-    // the 35 window words those four need (nine aligned 16-byte LDS reads)
-    // and applies all 32 coefficients with masks, independent operations
-    // instead of a serial loop over the set bits.
-    if (tid < MT_N / 4) {
-      const int i0 = 4 * tid;
+    // Thread t < 624 owns words 4 (t % 156) .. + 3 and coefficient words
+    // 2 (t / 156), + 1: per coefficient word it reads the 35 window words its
+    // four need (nine aligned 16-byte LDS reads) and applies all 32
+    // coefficients with masks, independent operations instead of a serial
+    // loop over the set bits; the four partial sums of a word meet in LDS
+    // XOR atomics.
+    if (tid < MT_N) {
+      const int i0 = 4 * (tid % (MT_N / 4));
+      const int part = tid / (MT_N / 4);
       uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-      for (int wq = 0; wq < MT_B / 32; wq++) {
+      for (int wq = 2 * part; wq < 2 * part + 2 && wq < MT_B / 32; wq++) {
         const int gw = q * (MT_B / 32) + wq;
         const uint32_t bits = (gw < MT_JUMP_WORDS) ? gl[gw] : 0u;
         if (!bits) continue;            // the same for every thread
@@ -103,20 +107,20 @@ __device__ void mt_jump_lds(const uint32_t *gl, uint32_t *S, uint32_t *acc,
           t3 ^= w[r + 3] & m;
         }
       }
-      acc[i0] ^= t0;
-      acc[i0 + 1] ^= t1;
-      acc[i0 + 2] ^= t2;
-      acc[i0 + 3] ^= t3;
+      if (t0) atomicXor(&acc[i0], t0);
+      if (t1) atomicXor(&acc[i0 + 1], t1);
+      if (t2) atomicXor(&acc[i0 + 2], t2);
+      if (t3) atomicXor(&acc[i0 + 3], t3);
     }
     __syncthreads();
   }
-  for (int i = tid; i < MT_N; i += MT_TPB) S[i] = (qtop >= 0) ? acc[i] : 0u;
+  for (int i = tid; i < MT_N; i += MT_JTPB) S[i] = (qtop >= 0) ? acc[i] : 0u;
   __syncthreads();
 }
 
 // One workgroup per output window o = c fan + k (o < nout):
 // out[o] = g_k(A) in[c], g_k = polys[k] (k = 0: a copy of in[c]).
-__global__ void __launch_bounds__(MT_TPB)
+__global__ void __launch_bounds__(MT_JTPB)
 k_mt_fan(const uint32_t *__restrict__ polys, const uint32_t *__restrict__ in,
     int fan, size_t nout, uint32_t *__restrict__ out) {
   __shared__ uint32_t gl[MT_JUMP_WORDS];
@@ -127,12 +131,12 @@ k_mt_fan(const uint32_t *__restrict__ polys, const uint32_t *__restrict__ in,
   if (o >= nout) return;                 // uniform over the workgroup
   const size_t c = o / fan;
   const int k = (int) (o % fan);
-  for (int i = tid; i < MT_N; i += MT_TPB) S[i] = in[c * MT_N + i];
+  for (int i = tid; i < MT_N; i += MT_JTPB) S[i] = in[c * MT_N + i];
   if (k > 0)
-    for (int i = tid; i < MT_JUMP_WORDS; i += MT_TPB) gl[i] = polys[(size_t) k * MT_JUMP_WORDS + i];
+    for (int i = tid; i < MT_JUMP_WORDS; i += MT_JTPB) gl[i] = polys[(size_t) k * MT_JUMP_WORDS + i];
   __syncthreads();
   if (k > 0) mt_jump_lds(gl, S, acc, nw);
-  for (int i = tid; i < MT_N; i += MT_TPB) out[o * MT_N + i] = S[i];
+  for (int i = tid; i < MT_N; i += MT_JTPB) out[o * MT_N + i] = S[i];
 }
 
 // One workgroup per segment s: words [s J, min((s + 1) J, n)) of the stream
@@ -224,7 +228,7 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
   if (first) {                           // window at word `first`
     const uint32_t *g = device_fan_polys(first, 2);
     if (!g) return fail(hipErrorInvalidValue);
-    hipLaunchKernelGGL(k_mt_fan, dim3(2), dim3(MT_TPB), 0, st, g, wa, 2, (size_t) 2, wb);
+    hipLaunchKernelGGL(k_mt_fan, dim3(2), dim3(MT_JTPB), 0, st, g, wa, 2, (size_t) 2, wb);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if ((e = hipMemcpyAsync(wa, wb + MT_N, MT_N * sizeof(uint32_t), hipMemcpyDeviceToDevice, st)) != hipSuccess)
       return fail(e);
@@ -237,7 +241,7 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
     const uint64_t need = (nseg + stride / J - 1) / (stride / J);
     const uint32_t *g = device_fan_polys(stride, MT_FAN);
     if (!g) return fail(hipErrorInvalidValue);
-    hipLaunchKernelGGL(k_mt_fan, dim3((unsigned) need), dim3(MT_TPB), 0, st, g, wa,
+    hipLaunchKernelGGL(k_mt_fan, dim3((unsigned) need), dim3(MT_JTPB), 0, st, g, wa,
         MT_FAN, (size_t) need, wb);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     uint32_t *t = wa; wa = wb; wb = t;
