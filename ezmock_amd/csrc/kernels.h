@@ -47,7 +47,6 @@ struct CicGeom {
   int ntx, nty, ntz;            // tiles per dimension
   size_t N;                     // particles (< 2^32)
   FastDiv fNg, fntz;            // division by Ng and by ntz
-  int sdi, sdj, sdk;            // lattice step of 256 particle indices
   // bounds blocks of the Lagrangian lattice (tile scan): BX x BY x BZ
   // lattice points, powers of two
   int BX, BY, BZ, lbx, lby, lbz;
