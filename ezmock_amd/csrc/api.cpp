@@ -102,8 +102,8 @@ struct Mesh {
   // power-spectrum estimator: two assignment grids, a Nyquist plane, sums
   real *pk_grid[2] = {nullptr, nullptr};
   // MT19937 stream words of the current stage (two per draw)
-  uint32_t *mtw = nullptr;
-  size_t mtw_cap = 0;
+  uint32_t *mtw = nullptr, *mtw2 = nullptr;   // MT19937 words (second: velocity draws)
+  size_t mtw_cap = 0, mtw2_cap = 0;
   // tracers attached to particles: particles per cell, particle classes,
   // tracers left per cell, block counts / offsets, contested cells and
   // particles, their selection and draws
@@ -218,7 +218,7 @@ void free_mesh_device(Mesh *m) {
   dfree(&m->d_num); dfree(&m->d_cnt); dfree(&m->d_res); dfree(&m->d_vb);
   dfree(&m->d_rsd);
   dfree(&m->pk_grid[0]); dfree(&m->pk_grid[1]); dfree(&m->pk_ny); dfree(&m->pk_acc);
-  dfree(&m->mtw); m->mtw_cap = 0;
+  dfree(&m->mtw); m->mtw_cap = 0; dfree(&m->mtw2); m->mtw2_cap = 0;
   dfree(&m->att_np); dfree(&m->att_bcnt); dfree(&m->att_cinfo); dfree(&m->att_plist);
   dfree(&m->att_ovf); dfree(&m->att_cls); dfree(&m->att_sel); dfree(&m->att_rest);
   for (int i = 0; i < 3; i++) dfree(&m->att_boff[i]);
@@ -547,20 +547,26 @@ int r2c_field(Mesh *m, int Ng, real *field) {
 
 // MT19937 instances: stream words of draws [first, first + n) of the
 // instance's seed into m->mtw (nullptr for MRG32k3a instances).
-int mt_words(EZMOCK *ez, Mesh *m, uint64_t first, uint64_t n, const uint32_t **out) {
+// MT19937 words of draws first .. first + n - 1 (two words per draw) into
+// word buffer `slot` (0 or 1) on stream st (default: the instance stream).
+int mt_words(EZMOCK *ez, Mesh *m, uint64_t first, uint64_t n, const uint32_t **out,
+    int slot = 0, hipStream_t st = nullptr) {
   *out = nullptr;
   if (RNG(ez)->type != 1 || n == 0) return 0;
-  if (m->mtw_cap < 2 * n) {
-    dfree(&m->mtw);
-    m->mtw_cap = 0;
-    int e = dalloc(&m->mtw, 2 * n);
+  uint32_t *&buf = slot ? m->mtw2 : m->mtw;
+  size_t &cap = slot ? m->mtw2_cap : m->mtw_cap;
+  if (!st) st = m->st;
+  if (cap < 2 * n) {
+    dfree(&buf);
+    cap = 0;
+    int e = dalloc(&buf, 2 * n);
     if (e) return e;
-    m->mtw_cap = 2 * n;
+    cap = 2 * n;
   }
-  Scope s(m, "mt_stream");
-  if (mt_stream_words(RNG(ez)->seed, 2 * first, 2 * n, m->mtw, m->st) != hipSuccess)
+  Scope s(m, "mt_stream", st);
+  if (mt_stream_words(RNG(ez)->seed, 2 * first, 2 * n, buf, st) != hipSuccess)
     return EZMOCK_ERR_RNG_JUMP;
-  *out = m->mtw;
+  *out = buf;
   return 0;
 }
 
@@ -951,6 +957,18 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   const int16_t *ntpos = m->nt;
   const uint64_t *offpos = m->blockoff;
   if (att_part && (e = attach_particles(ez, m, ntot, &iv, &ntpos, &offpos))) return e;
+  // MT19937: the velocity draws' words on a second stream while the
+  // position draws' words and the positions are made (the jump trees are
+  // latency-bound: the two overlap)
+  const bool vel_side = RNG(ez)->type == 1 && m->fft_streams && !m->profile;
+  const uint64_t nvel = 6 * (((uint64_t) ntot + 1) / 2);
+  const uint32_t *mtw_vel = nullptr;
+  if (vel_side) {
+    HIP_OK(hipEventRecord(m->ev_fork, m->st));
+    HIP_OK(hipStreamWaitEvent(m->sx[1], m->ev_fork, 0));
+    if ((e = mt_words(ez, m, 8 * (uint64_t) N, nvel, &mtw_vel, 1, m->sx[1]))) return e;
+    HIP_OK(hipEventRecord(m->ev_join[1], m->sx[1]));
+  }
   const uint32_t *mtw_pos = nullptr;
   if ((e = mt_words(ez, m, 5 * (uint64_t) N, 3 * (uint64_t) (ntot - iv), &mtw_pos))) return e;
   if (ntot > iv) {
@@ -997,8 +1015,8 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
       return e;
     m->mom_valid = !m->psi_ref;
   }
-  const uint32_t *mtw_vel = nullptr;
-  if ((e = mt_words(ez, m, 8 * (uint64_t) N, 6 * (((uint64_t) ntot + 1) / 2), &mtw_vel))) return e;
+  if (vel_side) HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[1], 0));
+  else if ((e = mt_words(ez, m, 8 * (uint64_t) N, nvel, &mtw_vel))) return e;
   {
     Scope s(m, "velocities");
     KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, iv, Ng, igs,

@@ -139,6 +139,15 @@ k_mt_fan(const uint32_t *__restrict__ polys, const uint32_t *__restrict__ in,
   for (int i = tid; i < MT_N; i += MT_JTPB) out[o * MT_N + i] = S[i];
 }
 
+// The seed window, on the device (one thread), so that a stream needs no
+// host synchronisation.
+__global__ void __launch_bounds__(64) k_mt_seed(uint64_t seed, uint32_t *__restrict__ w) {
+  __shared__ uint32_t sw[MT_N];
+  if (threadIdx.x == 0) mt_seed_window(seed, sw);
+  __syncthreads();
+  for (int i = threadIdx.x; i < MT_N; i += 64) w[i] = sw[i];
+}
+
 // One workgroup per segment s: words [s J, min((s + 1) J, n)) of the stream
 // whose window at word s J is starts[s].
 __global__ void __launch_bounds__(MT_TPB)
@@ -211,8 +220,7 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
   int L = 0;
   uint64_t cap = 1;
   while (cap < nseg) { cap *= MT_FAN; L++; }
-  uint32_t w0[MT_N];
-  mt_seed_window(seed, w0);
+
   // window buffers: level windows ping-pong, sized for nseg (+ fan slack)
   const size_t maxw = (size_t) nseg + MT_FAN;
   uint32_t *wa = nullptr, *wb = nullptr;
@@ -223,8 +231,8 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
     return e;
   }
   auto fail = [&](hipError_t er) { (void) hipFreeAsync(wa, st); (void) hipFreeAsync(wb, st); return er; };
-  if ((e = hipMemcpyAsync(wa, w0, sizeof(w0), hipMemcpyHostToDevice, st)) != hipSuccess) return fail(e);
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);  // w0 is on the host stack
+  hipLaunchKernelGGL(k_mt_seed, dim3(1), dim3(64), 0, st, seed, wa);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   if (first) {                           // window at word `first`
     const uint32_t *g = device_fan_polys(first, 2);
     if (!g) return fail(hipErrorInvalidValue);

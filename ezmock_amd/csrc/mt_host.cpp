@@ -155,30 +155,6 @@ void compute_phi(State &s) {
 
 }  // namespace
 
-void mt_seed_window(uint64_t seed, uint32_t *w) {
-  auto init = [&](uint32_t s0) {
-    w[0] = s0;
-    for (int i = 1; i < MT_N; i++)
-      w[i] = 1812433253u * (w[i - 1] ^ (w[i - 1] >> 30)) + (uint32_t) i;
-  };
-  if (seed <= 0xffffffffull) { init((uint32_t) seed); return; }
-  const uint32_t key[2] = {(uint32_t) (seed & 0xffffffffull), (uint32_t) (seed >> 32)};
-  init(19650218u);
-  int i = 1, j = 0;
-  for (int k = MT_N; k; k--) {
-    w[i] = (w[i] ^ ((w[i - 1] ^ (w[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t) j;
-    i++; j++;
-    if (i >= MT_N) { w[0] = w[MT_N - 1]; i = 1; }
-    if (j >= 2) j = 0;
-  }
-  for (int k = MT_N - 1; k; k--) {
-    w[i] = (w[i] ^ ((w[i - 1] ^ (w[i - 1] >> 30)) * 1566083941u)) - (uint32_t) i;
-    i++;
-    if (i >= MT_N) { w[0] = w[MT_N - 1]; i = 1; }
-  }
-  w[0] = 0x80000000u;
-}
-
 const uint32_t *mt_jump_poly(uint64_t e) {
   std::call_once(g_once, [] { compute_phi(st()); });
   State &s = st();
