@@ -65,6 +65,68 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 
 #define GEN_CHUNK (sizeof(real) == 4 ? 16 : 8)     // cells per thread
 
+// One cell (i, j, k) of the generated spectrum from its two draws (amp in
+// (0, 1], phase draw ph01 in [0, 1)), perturb.c:78-359: whether this cell
+// is written (cells of the kz = 0 and Nyquist planes outside the lower half
+// are filled by their mirror), whether it also writes its mirror (ni, nj),
+// and its value.
+struct GenCell {
+  real re, im;
+  bool write, mirror;
+  int ni, nj;
+};
+
+__device__ __forceinline__ GenCell gen_cell(int i, int j, int k, double amp, double ph01,
+    const GenParams &gp, const double *__restrict__ ptab) {
+  const int Ng = gp.Ng, Ngh = Ng >> 1;
+  GenCell c{0, 0, true, false, 0, 0};
+  double ph = ph01 * 2 * EZ_PI;
+  const bool plane_k = (k == 0) || (k == Ngh);
+  if (plane_k) {
+    c.ni = (i == 0) ? 0 : Ng - i;
+    c.nj = (j == 0) ? 0 : Ng - j;
+    const bool lower = (i == c.ni && j <= Ngh) || (i != c.ni && i <= Ngh);
+    if (!lower) c.write = false;      // filled by its mirror
+    c.mirror = lower;
+  }
+  if (!c.write) return c;
+  const int ki = (i <= Ngh) ? i : i - Ng;
+  const int kj = (j <= Ngh) ? j : j - Ng;
+  const int ksqi = ki * ki + kj * kj + k * k;
+  const double ksq = (double) ksqi;
+  double P = ptab[ksqi];
+  double a = gp.fixamp ? 1.0 : -log(amp);
+  if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
+  else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
+  P *= gp.Pfac;
+  if (gp.iphase) ph += EZ_PI;
+  P /= (ksq * gp.kfac);
+  double sn, cs;
+  sincos(ph, &sn, &cs);                 // one argument reduction for both
+  c.re = (real) (-P * sn);
+  c.im = (real) (P * cs);
+  // perturb.c:341-359: self-conjugate modes keep only the imaginary part
+  // (the reference writes (+P sin, P cos) twice, then zeroes re).
+  if (c.mirror && i == c.ni && j == c.nj) c.re = 0;
+  return c;
+}
+
+// The mirror of a written kz = 0 / Nyquist cell (not for self-conjugate
+// modes): (-re, im) at (ni, nj, k).
+__device__ __forceinline__ void gen_mirror(const GenCell &c, int i, int j, int k,
+    const GenParams &gp, cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
+    real *__restrict__ ref_copy) {
+  if (!c.mirror || (i == c.ni && j == c.nj)) return;
+  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
+  const cplx w = {-c.re, c.im};
+  if (k < Ngh) rhok[((size_t) c.ni * Ng + c.nj) * Ngh + k] = w;
+  else rhok_ny[(size_t) c.ni * Ng + c.nj] = w;
+  if (ref_copy) {
+    const size_t nidx = ((size_t) c.ni * Ng + c.nj) * Ngk + k;
+    ref_copy[2 * nidx] = -c.re; ref_copy[2 * nidx + 1] = c.im;
+  }
+}
+
 // MRG32k3a state at the first draw of each k_gen_rhok workgroup, one thread
 // per workgroup, so that no workgroup waits on a serial jump of its own.
 __global__ void __launch_bounds__(256)
@@ -78,12 +140,11 @@ k_gen_base(MrgState seed, uint64_t mult, size_t nblk, size_t cells_per_blk,
   base[b] = g;
 }
 
-template <int CH, bool MT>
+template <int CH>
 __global__ void __launch_bounds__(GEN_TPB)
 k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
     real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
-    const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
-    const MrgState *__restrict__ bases) {
+    const uint32_t *__restrict__ jtab, const MrgState *__restrict__ bases) {
   constexpr int NB = GEN_TPB * CH;
   __shared__ cplx sv[NB + NB / CH];             // one pad slot per thread
   __shared__ uint16_t so[NB];
@@ -97,79 +158,32 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   int i = (int) (row / (size_t) Ng);
   int j = (int) (row - (size_t) i * Ng);
   const uint64_t mult = gp.fixamp ? 1 : 2;
-  uint64_t d = (idx0 == 0 ? 0 : (idx0 - 1)) * mult;   // next draw (MT19937 words)
-  // MRG32k3a: the workgroup's state at the draw of its first cell (from
+  const uint64_t d = (idx0 == 0 ? 0 : (idx0 - 1)) * mult;   // first draw
+  // the workgroup's MRG32k3a state at the draw of its first cell (from
   // k_gen_base), then a short jump per thread (< 2 base-256 digits)
   const uint64_t dblk = (blk0 == 0 ? 0 : (blk0 - 1)) * mult;
-  MrgState g;
-  if (!MT) {
-    g = bases[blockIdx.x];
-    if (idx0 < gp.Nh) mrg_jump(g, d - dblk, jtab);
-  }
+  MrgState g = bases[blockIdx.x];
+  if (idx0 < gp.Nh) mrg_jump(g, d - dblk, jtab);
   for (int c = 0; c < CH; c++) {
     const size_t idx = idx0 + c;
     const int e = threadIdx.x * CH + c;
     uint16_t off = GEN_SKIP;
     if (idx < gp.Nh) {
-      real re = 0, im = 0;
-      bool write = true, mirror = false;
-      int ni = 0, nj = 0;
+      GenCell gc{0, 0, true, false, 0, 0};
       if (idx != 0) {
-        double amp, ph;
-        if (MT) {
-          amp = gp.fixamp ? 1.0 : mt_draw_pos(mtw, d++);
-          ph = mt_draw(mtw, d++) * 2 * EZ_PI;
-        }
-        else {
-          amp = gp.fixamp ? 1.0 : mrg_next(g);
-          ph = mrg_next(g) * 2 * EZ_PI;
-        }
-        const bool plane_k = (k == 0) || (k == Ngh);
-        if (plane_k) {
-          ni = (i == 0) ? 0 : Ng - i;
-          nj = (j == 0) ? 0 : Ng - j;
-          const bool lower = (i == ni && j <= Ngh) || (i != ni && i <= Ngh);
-          if (!lower) write = false;    // filled by its mirror
-          mirror = lower;
-        }
-        if (write) {
-          const int ki = (i <= Ngh) ? i : i - Ng;
-          const int kj = (j <= Ngh) ? j : j - Ng;
-          const int ksqi = ki * ki + kj * kj + k * k;
-          const double ksq = (double) ksqi;
-          double P = ptab[ksqi];
-          double a = gp.fixamp ? 1.0 : -log(amp);
-          if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
-          else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
-          P *= gp.Pfac;
-          if (gp.iphase) ph += EZ_PI;
-          P /= (ksq * gp.kfac);
-          double sn, cs;
-          sincos(ph, &sn, &cs);           // one argument reduction for both
-          re = (real) (-P * sn);
-          im = (real) (P * cs);
-          // perturb.c:341-359: self-conjugate modes keep only the imaginary
-          // part (the reference writes (+P sin, P cos) twice, then zeroes re).
-          if (mirror && i == ni && j == nj) re = 0;
-        }
+        const double amp = gp.fixamp ? 1.0 : mrg_next(g);
+        const double ph = mrg_next(g);
+        gc = gen_cell(i, j, k, amp, ph, gp, ptab);
       }
-      if (write) {
-        const cplx v = {re, im};
+      if (gc.write) {
+        const cplx v = {gc.re, gc.im};
         if (k < Ngh) {
           sv[e + e / CH] = v;
           off = (uint16_t) (idx - row - obase);
         }
         else rhok_ny[row] = v;
-        if (ref_copy) { ref_copy[2 * idx] = re; ref_copy[2 * idx + 1] = im; }
-        if (mirror && !(i == ni && j == nj)) {
-          const cplx w = {-re, im};
-          if (k < Ngh) rhok[((size_t) ni * Ng + nj) * Ngh + k] = w;
-          else rhok_ny[(size_t) ni * Ng + nj] = w;
-          if (ref_copy) {
-            const size_t nidx = ((size_t) ni * Ng + nj) * Ngk + k;
-            ref_copy[2 * nidx] = -re; ref_copy[2 * nidx + 1] = im;
-          }
-        }
+        if (ref_copy) { ref_copy[2 * idx] = gc.re; ref_copy[2 * idx + 1] = gc.im; }
+        gen_mirror(gc, i, j, k, gp, rhok, rhok_ny, ref_copy);
       }
     }
     so[e] = off;
@@ -182,6 +196,41 @@ k_gen_rhok(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
   for (int e = threadIdx.x; e < NB; e += GEN_TPB) {
     const uint16_t o = so[e];
     if (o != GEN_SKIP) rhok[obase + o] = sv[e + e / CH];
+  }
+}
+
+// MT19937: the draws come from a buffer of stream words, so a thread's
+// cells need not be consecutive.  Lane-consecutive cells make the word
+// reads and the spectrum stores coalesced, with no LDS staging (per-thread
+// runs of 16 cells read each word load as 64 separate lines).
+template <int CH>
+__global__ void __launch_bounds__(GEN_TPB)
+k_gen_rhok_mt(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
+    real *__restrict__ ref_copy, GenParams gp, const double *__restrict__ ptab,
+    const uint32_t *__restrict__ mtw) {
+  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
+  const uint64_t mult = gp.fixamp ? 1 : 2;
+  const size_t blk0 = (size_t) blockIdx.x * GEN_TPB * CH;
+  for (int c = 0; c < CH; c++) {
+    const size_t idx = blk0 + (size_t) c * GEN_TPB + threadIdx.x;
+    if (idx >= gp.Nh) break;
+    const size_t row = idx / (size_t) Ngk;
+    const int k = (int) (idx - row * (size_t) Ngk);
+    const int i = (int) (row / (size_t) Ng);
+    const int j = (int) (row - (size_t) i * Ng);
+    GenCell gc{0, 0, true, false, 0, 0};
+    if (idx != 0) {
+      uint64_t d = (idx - 1) * mult;
+      const double amp = gp.fixamp ? 1.0 : mt_draw_pos(mtw, d++);
+      const double ph = mt_draw(mtw, d);
+      gc = gen_cell(i, j, k, amp, ph, gp, ptab);
+    }
+    if (!gc.write) continue;
+    const cplx v = {gc.re, gc.im};
+    if (k < Ngh) rhok[idx - row] = v;
+    else rhok_ny[row] = v;
+    if (ref_copy) { ref_copy[2 * idx] = gc.re; ref_copy[2 * idx + 1] = gc.im; }
+    gen_mirror(gc, i, j, k, gp, rhok, rhok_ny, ref_copy);
   }
 }
 
@@ -198,15 +247,15 @@ hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
   const size_t nthr = (gp.Nh + CH - 1) / CH;
   const size_t nblk = ez_blocks(nthr, GEN_TPB);
   if (mtw)
-    hipLaunchKernelGGL((k_gen_rhok<CH, true>), dim3(nblk), dim3(GEN_TPB), 0,
-        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw, (const MrgState *) nullptr);
+    hipLaunchKernelGGL((k_gen_rhok_mt<CH>), dim3(nblk), dim3(GEN_TPB), 0, st,
+        rhok, rhok_ny, ref_copy, gp, ptab, mtw);
   else {
     if (!bases) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gen_base, dim3(ez_blocks(nblk, 256)), dim3(256), 0, st,
         gp.seed_state, (uint64_t) (gp.fixamp ? 1 : 2), nblk, (size_t) GEN_TPB * CH, jtab,
         bases);
-    hipLaunchKernelGGL((k_gen_rhok<CH, false>), dim3(nblk), dim3(GEN_TPB), 0,
-        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, mtw, (const MrgState *) bases);
+    hipLaunchKernelGGL((k_gen_rhok<CH>), dim3(nblk), dim3(GEN_TPB), 0,
+        st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, (const MrgState *) bases);
   }
   return hipGetLastError();
 }
