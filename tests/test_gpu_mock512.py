@@ -47,7 +47,7 @@ def gpu_mock(pk, prec, seed=SEED, keep_spec=False):
     return ez
 
 
-def test_config2_noise_density_and_oracle(gpu_available, pk_table):
+def test_mock512_noise_density_and_oracle(gpu_available, pk_table):
     k, P = pk_table
     threads = len(os.sched_getaffinity(0))
     o = po.EZmockOracle(Lbox=L, Ngrid=NG, rng=0, seed=SEED, nthread=threads, prec="f32",
@@ -75,7 +75,7 @@ def test_config2_noise_density_and_oracle(gpu_available, pk_table):
     cg = g32.device_catalog()
     n_pre = pc.identical_before_first_difference(ntg, cg, nto, co, 2e-5)
     frac = pc.identical_tracer_fraction(ntg, cg, nto, co, 2e-5)
-    print(f"config 2: FP32 spectrum bit-identical to the FP32 oracle in {same_spec:.6f} "
+    print(f"512^3 mock: FP32 spectrum bit-identical to the FP32 oracle in {same_spec:.6f} "
           f"of the values (max rel diff {rel_spec:.2e}); density rel RMS FP32-FP64 GPU "
           f"{rms:.3e}, FP32 GPU - FP32 oracle {rms_o:.3e}; tracers per cell identical to "
           f"the FP32 oracle in {same_nt:.6f} of the cells; {n_pre} of {len(co[0])} tracers "
@@ -93,7 +93,7 @@ def test_config2_noise_density_and_oracle(gpu_available, pk_table):
     o.close()
 
 
-def test_config2_fp64_noise_against_oracle(gpu_available, pk_table):
+def test_mock512_fp64_noise_against_oracle(gpu_available, pk_table):
     k, P = pk_table
     threads = len(os.sched_getaffinity(0))
     o = po.EZmockOracle(Lbox=L, Ngrid=NG, rng=0, seed=SEED, nthread=threads, prec="f64",
@@ -104,13 +104,13 @@ def test_config2_fp64_noise_against_oracle(gpu_available, pk_table):
     g = gpu_mock(pk_table, "f64", keep_spec=True)
     sg, so = g.field("spec"), o.field("spec")
     rel = np.abs(sg - so).max() / np.abs(so).max()
-    print(f"config 2: FP64 spectrum max rel diff to the FP64 oracle {rel:.2e}, "
+    print(f"512^3 mock: FP64 spectrum max rel diff to the FP64 oracle {rel:.2e}, "
           f"bit-identical in {np.mean(sg == so):.6f}")
     assert rel <= 4e-16
     o.close()
 
 
-def test_config2_multipoles_fp32_vs_fp64_ensemble(gpu_available, pk_table):
+def test_mock512_multipoles_fp32_vs_fp64_ensemble(gpu_available, pk_table):
     seeds = list(range(42, 58))
     res = {"f32": [], "f64": []}
     for prec in ("f64", "f32"):
@@ -127,7 +127,7 @@ def test_config2_multipoles_fp32_vs_fp64_ensemble(gpu_available, pk_table):
     d0 = np.abs(P0["f32"] - P0["f64"]) / P0["f64"]
     d2 = np.abs(P2["f32"] - P2["f64"]) / P0["f64"]
     d2r = np.abs(P2["f32"] - P2["f64"]) / np.abs(P2["f64"])
-    print(f"config 2 multipoles, 0.02 <= k < 0.4: one seed max |dP0|/P0 {one0.max():.4f}; "
+    print(f"512^3 mock multipoles, 0.02 <= k < 0.4: one seed max |dP0|/P0 {one0.max():.4f}; "
           f"mean of {len(seeds)} seeds: max |dP0|/P0 {d0.max():.4f}, max |dP2|/P0 "
           f"{d2.max():.4f}, max |dP2|/|P2| {d2r.max():.4f}")
     assert d0.max() <= 0.005

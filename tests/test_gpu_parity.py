@@ -232,7 +232,7 @@ def test_unsupported_configurations_fail_loudly(gpu_available, pk_table):
         g.create_dens_field_from_ic()
 
 
-def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
+def test_mock256_fp64_against_oracle(gpu_available, pk_table):
     """Plain EZmock single mock, FP64, as the reference CPU path runs it:
     256^3, Lbox 1000, z 0.5, MRG32k3a seed 1, (0.5, 10, 0.3, 150),
     3e-4 (h/Mpc)^3 -> 300000 expected tracers."""
@@ -245,7 +245,7 @@ def test_config1_256_fp64_against_oracle(gpu_available, pk_table):
     pg, pO = g.pdf_info(), o.pdf_info()
     assert abs(pg["ntot"] - pO["ntot"]) <= 1e-4 * pO["ntot"]
     frac = pc.identical_tracer_fraction(ntg, cg, nto, co, 1e-9)
-    print(f"config 1: identical tracers {frac:.6f} of {len(co[0])}, ntot {pg['ntot']} "
+    print(f"256^3 FP64 mock: identical tracers {frac:.6f} of {len(co[0])}, ntot {pg['ntot']} "
           f"vs {pO['ntot']}")
     assert frac >= 0.9999
 
