@@ -280,6 +280,13 @@ class EZmock:
         self._L.EZmock_ext_pk_table(self._ez, k.ctypes.data, P.ctypes.data, P2.ctypes.data, n)
         return k, P, P2
 
+    def set_seed(self, seed):
+        """Seed of the random streams of the next field and catalogues (the
+        instance is kept: its device buffers, FFT schedule and, for MT19937,
+        jump windows of an unchanged seed are reused)."""
+        self._L.EZmock_ext_set_seed(self._ez, int(seed), C.byref(self._err))
+        self._assert_return_value()
+
     def set_option(self, option, value):
         return self._L.EZmock_ext_set_option(self._ez, int(option), int(value))
 

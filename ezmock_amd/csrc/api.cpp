@@ -104,6 +104,17 @@ struct Mesh {
   // MT19937 stream words of the current stage (two per draw)
   uint32_t *mtw = nullptr, *mtw2 = nullptr;   // MT19937 words (second: velocity draws)
   size_t mtw_cap = 0, mtw2_cap = 0;
+  // MT19937 segment windows per stream use (0 spectrum, 1 bias, 2
+  // positions, 3 velocities, 4 particle selection), keyed by seed and first draw: the jump trees
+  // depend on nothing else, so they are kept across fields and built ahead
+  // of use on smt where the draw counts are bounded in advance
+  struct MtWin {
+    uint32_t *win = nullptr, *scr = nullptr;     // windows; the tree's scratch
+    size_t cap = 0, nseg = 0;
+    uint64_t seed = 0, first = ~0ull;
+    hipEvent_t ready = nullptr, used = nullptr;   // built; last read
+  } mtwin[5];
+  hipStream_t smt = nullptr;
   // tracers attached to particles: particles per cell, particle classes,
   // tracers left per cell, block counts / offsets, contested cells and
   // particles, their selection and draws
@@ -160,6 +171,7 @@ int device_ready(Mesh *m) {
     HIP_OK(hipStreamCreateWithFlags(&m->sx[i], hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&m->ev_join[i], hipEventDisableTiming));
   }
+  HIP_OK(hipStreamCreateWithFlags(&m->smt, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&m->ev_pos, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&m->ev_xyz, hipEventDisableTiming));
@@ -203,6 +215,7 @@ void free_tracer_scratch(Mesh *m) {
 }
 
 void free_mesh_device(Mesh *m) {
+  if (m->smt) (void) hipStreamSynchronize(m->smt);   // window builds ahead of use
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
   dfree(&m->vel4); dfree(&m->gen_base);
   dfree(&m->rho); dfree(&m->ny); dfree(&m->slot0); dfree(&m->rhot); dfree(&m->nt);
@@ -238,6 +251,17 @@ void free_mesh_device(Mesh *m) {
   }
   if (m->ev_fork) (void) hipEventDestroy(m->ev_fork);
   m->ev_fork = nullptr;
+  for (auto &w : m->mtwin) {
+    dfree(&w.win);
+    dfree(&w.scr);
+    w.cap = w.nseg = 0;
+    w.first = ~0ull;
+    if (w.ready) (void) hipEventDestroy(w.ready);
+    if (w.used) (void) hipEventDestroy(w.used);
+    w.ready = w.used = nullptr;
+  }
+  if (m->smt) (void) hipStreamDestroy(m->smt);
+  m->smt = nullptr;
   if (m->ev_pos) (void) hipEventDestroy(m->ev_pos);
   if (m->ev_xyz) (void) hipEventDestroy(m->ev_xyz);
   m->ev_pos = m->ev_xyz = nullptr;
@@ -545,11 +569,47 @@ int r2c_field(Mesh *m, int Ng, real *field) {
   return 0;
 }
 
-// MT19937 instances: stream words of draws [first, first + n) of the
-// instance's seed into m->mtw (nullptr for MRG32k3a instances).
+// MT19937 segment windows of draws [first, first + n) for stream use k,
+// kept from an earlier build when seed and first match and it covers n
+// draws; otherwise built on stream st, whose completion `ready` marks.
+int mt_windows(EZMOCK *ez, Mesh *m, int k, uint64_t first, uint64_t n, hipStream_t st) {
+  Mesh::MtWin &w = m->mtwin[k];
+  const uint64_t seed = RNG(ez)->seed;
+  const size_t nseg = mt_segments(2 * n);
+  if (w.seed == seed && w.first == first && w.nseg >= nseg) return 0;
+  if (!w.ready) {
+    HIP_OK(hipEventCreateWithFlags(&w.ready, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&w.used, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(w.used, st));
+  }
+  HIP_OK(hipStreamWaitEvent(st, w.used, 0));   // the last fill from them is done
+  if (w.cap < nseg) {
+    HIP_OK(hipEventSynchronize(w.used));
+    dfree(&w.win);
+    dfree(&w.scr);
+    w.cap = 0;
+    int e = dalloc(&w.win, mt_window_words(nseg));
+    if (!e) e = dalloc(&w.scr, mt_window_words(nseg));
+    if (e) return e;
+    w.cap = nseg;
+  }
+  w.first = ~0ull;                      // invalid until the build is enqueued
+  {
+    Scope s(m, "mt_stream", st);
+    if (mt_stream_windows(seed, 2 * first, nseg, w.win, w.scr, st) != hipSuccess)
+      return EZMOCK_ERR_RNG_JUMP;
+  }
+  HIP_OK(hipEventRecord(w.ready, st));
+  w.seed = seed;
+  w.first = first;
+  w.nseg = nseg;
+  return 0;
+}
+
 // MT19937 words of draws first .. first + n - 1 (two words per draw) into
-// word buffer `slot` (0 or 1) on stream st (default: the instance stream).
-int mt_words(EZMOCK *ez, Mesh *m, uint64_t first, uint64_t n, const uint32_t **out,
+// word buffer `slot` (0 or 1) on stream st (default: the instance stream),
+// from the windows of stream use k.
+int mt_words(EZMOCK *ez, Mesh *m, int k, uint64_t first, uint64_t n, const uint32_t **out,
     int slot = 0, hipStream_t st = nullptr) {
   *out = nullptr;
   if (RNG(ez)->type != 1 || n == 0) return 0;
@@ -563,9 +623,15 @@ int mt_words(EZMOCK *ez, Mesh *m, uint64_t first, uint64_t n, const uint32_t **o
     if (e) return e;
     cap = 2 * n;
   }
-  Scope s(m, "mt_stream", st);
-  if (mt_stream_words(RNG(ez)->seed, 2 * first, 2 * n, buf, st) != hipSuccess)
-    return EZMOCK_ERR_RNG_JUMP;
+  int e = mt_windows(ez, m, k, first, n, st);
+  if (e) return e;
+  HIP_OK(hipStreamWaitEvent(st, m->mtwin[k].ready, 0));
+  {
+    Scope s(m, "mt_stream", st);
+    if (mt_stream_fill(m->mtwin[k].win, 2 * n, buf, st) != hipSuccess)
+      return EZMOCK_ERR_RNG_JUMP;
+  }
+  HIP_OK(hipEventRecord(m->mtwin[k].used, st));
   *out = buf;
   return 0;
 }
@@ -700,7 +766,7 @@ int attach_particles(EZMOCK *ez, Mesh *m, size_t ntot, size_t *iv,
     // (pop_tracer.c:781, 846)
     if (RNG(ez)->type == 1) {
       const uint32_t *w = nullptr;
-      if ((e = mt_words(ez, m, 4 * (uint64_t) N, ncont, &w))) return e;
+      if ((e = mt_words(ez, m, 4, 4 * (uint64_t) N, ncont, &w))) return e;
       KCHK(launch_uniform_mt(w, ncont, 0, m->att_draws, m->st));
     }
     else {
@@ -825,6 +891,16 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   m->ntot0 = ntot;
 
   if ((e = device_ready(m))) return e;
+  if (RNG(ez)->type == 1) {
+    // the position and velocity draws' windows on their own stream while
+    // the bias model runs: at most 3 and 6 per tracer pair, with a margin
+    // over the expected count (a larger final count rebuilds them)
+    const uint64_t nb = ntot + ntot / 32 + 4096;
+    hipStream_t sp = m->profile ? m->st : m->smt;   // timed runs: one stream
+    if ((e = mt_windows(ez, m, 2, 5 * (uint64_t) N, 3 * nb, sp)) ||
+        (e = mt_windows(ez, m, 3, 8 * (uint64_t) N, 6 * ((nb + 1) / 2), sp)))
+      return e;
+  }
   if (m->psi_ref) {             // the reference reads the caller's arrays
     for (int i = 0; i < 3; i++)
       HIP_OK(hipMemcpyAsync(m->psi[i], m->psi_host[i], N * sizeof(real),
@@ -871,7 +947,8 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if (nvalid < ncell) return EZMOCK_ERR_RHO_LESS;
   KCHK(launch_zero(m->hist, EZ_NUM_LOGB_BIN * sizeof(unsigned long long), m->st));
   const uint32_t *mtw_bias = nullptr;
-  if ((e = mt_words(ez, m, 2 * (uint64_t) N, 2 * ((nvalid + 1) / 2), &mtw_bias))) return e;
+  if ((e = mt_words(ez, m, 1, 2 * (uint64_t) N, 2 * ((nvalid + 1) / 2), &mtw_bias)))
+    return e;
   {
     Scope s(m, "bias");
     KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, m->blockhist, bp,
@@ -966,11 +1043,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if (vel_side) {
     HIP_OK(hipEventRecord(m->ev_fork, m->st));
     HIP_OK(hipStreamWaitEvent(m->sx[1], m->ev_fork, 0));
-    if ((e = mt_words(ez, m, 8 * (uint64_t) N, nvel, &mtw_vel, 1, m->sx[1]))) return e;
+    if ((e = mt_words(ez, m, 3, 8 * (uint64_t) N, nvel, &mtw_vel, 1, m->sx[1]))) return e;
     HIP_OK(hipEventRecord(m->ev_join[1], m->sx[1]));
   }
   const uint32_t *mtw_pos = nullptr;
-  if ((e = mt_words(ez, m, 5 * (uint64_t) N, 3 * (uint64_t) (ntot - iv), &mtw_pos))) return e;
+  if ((e = mt_words(ez, m, 2, 5 * (uint64_t) N, 3 * (uint64_t) (ntot - iv), &mtw_pos)))
+    return e;
   if (ntot > iv) {
     Scope s(m, "positions");
     KCHK(launch_positions(ntpos, offpos, Ng, conf->Lbox, gs, bp.seed_state,
@@ -1016,7 +1094,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     m->mom_valid = !m->psi_ref;
   }
   if (vel_side) HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[1], 0));
-  else if ((e = mt_words(ez, m, 8 * (uint64_t) N, nvel, &mtw_vel))) return e;
+  else if ((e = mt_words(ez, m, 3, 8 * (uint64_t) N, nvel, &mtw_vel))) return e;
   {
     Scope s(m, "velocities");
     KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, iv, Ng, igs,
@@ -1353,7 +1431,13 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     }
     else {
       const uint32_t *mtw = nullptr;
-      if ((e = mt_words(ez, m, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw))) return (*err = e);
+      if ((e = mt_words(ez, m, 0, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw)))
+        return (*err = e);
+      // the bias draws' windows (at most N + 1 draws from 2N) on their own
+      // stream while the field is made
+      if (RNG(ez)->type == 1 && (e = mt_windows(ez, m, 1, 2 * (uint64_t) N, (uint64_t) N + 1,
+          m->profile ? m->st : m->smt)))
+        return (*err = e);
       if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return (*err = e);
       Scope s(m, "gen_spectrum");
       KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
@@ -1805,14 +1889,17 @@ int EZmock_ext_uniform(const int randgen, const uint64_t seed,
     if (launch_uniform(s, start, n, tmp.jtab, d, tmp.st) != hipSuccess) e = EZMOCK_ERR_UNKNOWN;
   }
   else {                        // MT19937: two output words per draw
-    uint32_t *w = nullptr;
-    if ((e = dalloc(&w, 2 * n)) == 0) {
-      if (mt_stream_words(seed, 2 * start, 2 * n, w, tmp.st) != hipSuccess ||
+    uint32_t *w = nullptr, *win = nullptr;
+    const size_t nseg = mt_segments(2 * n), ww = mt_window_words(nseg);
+    if ((e = dalloc(&w, 2 * n)) == 0 && (e = dalloc(&win, 2 * ww)) == 0) {
+      if (mt_stream_windows(seed, 2 * start, nseg, win, win + ww, tmp.st) != hipSuccess ||
+          mt_stream_fill(win, 2 * n, w, tmp.st) != hipSuccess ||
           launch_uniform_mt(w, n, pos ? 1 : 0, d, tmp.st) != hipSuccess)
         e = EZMOCK_ERR_RNG_JUMP;
       if (hipStreamSynchronize(tmp.st) != hipSuccess) e = EZMOCK_ERR_UNKNOWN;
-      dfree(&w);
     }
+    dfree(&w);
+    dfree(&win);
   }
   if (!e && (hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, tmp.st) != hipSuccess ||
       hipStreamSynchronize(tmp.st) != hipSuccess))

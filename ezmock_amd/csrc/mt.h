@@ -34,7 +34,16 @@ __host__ __device__ inline void mt_seed_window(uint64_t seed, uint32_t *w) {
 // cached; nullptr if the characteristic polynomial failed its check.
 const uint32_t *mt_jump_poly(uint64_t e);
 
-// Words first .. first + n - 1 of the tempered output stream of `seed`
-// into out (device, n u32), on stream st.  Returns a HIP error code.
-hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
-    uint32_t *out, hipStream_t st);
+// Words first .. first + n - 1 of the tempered output stream of `seed`, in
+// two steps, so that the segment windows of a stream (the jump trees:
+// latency-bound, seed-dependent only) can be built ahead of time, on another
+// stream, and reused: nseg windows at words first + s J (win and the tree's
+// scratch scr: mt_window_words(nseg) u32 each, owned by the caller, so that
+// builds on several streams share no stream-ordered pool memory), then the
+// words [first, first + n) into out (device, n u32) from windows built for
+// at least mt_segments(n) segments.  Both return a HIP error code.
+size_t mt_segments(uint64_t n);
+size_t mt_window_words(size_t nseg);
+hipError_t mt_stream_windows(uint64_t seed, uint64_t first, size_t nseg, uint32_t *win,
+    uint32_t *scr, hipStream_t st);
+hipError_t mt_stream_fill(const uint32_t *win, uint64_t n, uint32_t *out, hipStream_t st);

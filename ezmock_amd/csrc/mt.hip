@@ -212,34 +212,33 @@ const uint32_t *device_fan_polys(uint64_t stride, int fan) {
 }
 }  // namespace
 
-hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
-    uint32_t *out, hipStream_t st) {
-  if (n == 0) return hipSuccess;
+size_t mt_segments(uint64_t n) { return (size_t) ((n + MT_SEG_WORDS - 1) / MT_SEG_WORDS); }
+
+size_t mt_window_words(size_t nseg) { return (nseg + MT_FAN) * MT_N; }
+
+// Windows of the stream at words first + s J, s < nseg, into win, with scr
+// as the other half of the ping-pong (both mt_window_words(nseg) words): the
+// seed window, one jump to `first`, then log16(nseg) levels of fan-out jumps.
+hipError_t mt_stream_windows(uint64_t seed, uint64_t first, size_t nseg, uint32_t *win,
+    uint32_t *scr, hipStream_t st) {
+  if (nseg == 0) return hipSuccess;
   const uint64_t J = MT_SEG_WORDS;
-  const uint64_t nseg = (n + J - 1) / J;
   int L = 0;
   uint64_t cap = 1;
   while (cap < nseg) { cap *= MT_FAN; L++; }
-
-  // window buffers: level windows ping-pong, sized for nseg (+ fan slack)
-  const size_t maxw = (size_t) nseg + MT_FAN;
-  uint32_t *wa = nullptr, *wb = nullptr;
   hipError_t e;
-  if ((e = hipMallocAsync((void **) &wa, maxw * MT_N * sizeof(uint32_t), st)) != hipSuccess) return e;
-  if ((e = hipMallocAsync((void **) &wb, maxw * MT_N * sizeof(uint32_t), st)) != hipSuccess) {
-    (void) hipFreeAsync(wa, st);
-    return e;
-  }
-  auto fail = [&](hipError_t er) { (void) hipFreeAsync(wa, st); (void) hipFreeAsync(wb, st); return er; };
+  // levels ping-pong between win and scr; the root goes where the last
+  // level ends in win
+  uint32_t *wa = (L % 2 == 0) ? win : scr, *wb = (L % 2 == 0) ? scr : win;
   hipLaunchKernelGGL(k_mt_seed, dim3(1), dim3(64), 0, st, seed, wa);
-  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   if (first) {                           // window at word `first`
     const uint32_t *g = device_fan_polys(first, 2);
-    if (!g) return fail(hipErrorInvalidValue);
+    if (!g) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_mt_fan, dim3(2), dim3(MT_JTPB), 0, st, g, wa, 2, (size_t) 2, wb);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(wa, wb + MT_N, MT_N * sizeof(uint32_t), hipMemcpyDeviceToDevice, st)) != hipSuccess)
-      return fail(e);
+      return e;
   }
   // level l: windows at stride J FAN^(L-1-l), each one jump x^(k stride)
   // (k < FAN) from a window of the coarser level
@@ -248,15 +247,20 @@ hipError_t mt_stream_words(uint64_t seed, uint64_t first, uint64_t n,
     for (int k = 0; k < L - 1 - l; k++) stride *= MT_FAN;
     const uint64_t need = (nseg + stride / J - 1) / (stride / J);
     const uint32_t *g = device_fan_polys(stride, MT_FAN);
-    if (!g) return fail(hipErrorInvalidValue);
+    if (!g) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_mt_fan, dim3((unsigned) need), dim3(MT_JTPB), 0, st, g, wa,
         MT_FAN, (size_t) need, wb);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     uint32_t *t = wa; wa = wb; wb = t;
   }
-  hipLaunchKernelGGL(k_mt_fill, dim3((unsigned) nseg), dim3(MT_TPB), 0, st, wa, J, n, out);
-  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-  (void) hipFreeAsync(wa, st);
-  (void) hipFreeAsync(wb, st);
   return hipSuccess;
+}
+
+// Words [first, first + n) into out from the windows win of
+// mt_stream_windows(seed, first, nseg >= mt_segments(n)).
+hipError_t mt_stream_fill(const uint32_t *win, uint64_t n, uint32_t *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mt_fill, dim3((unsigned) mt_segments(n)), dim3(MT_TPB), 0, st, win,
+      (uint64_t) MT_SEG_WORDS, n, out);
+  return hipGetLastError();
 }

@@ -207,6 +207,69 @@ def test_repeated_populate_and_determinism(gpu_available, pk_table):
     assert len(b[0]) != len(a[0]) or not np.array_equal(a[0], b[0])
 
 
+def test_mt19937_instance_reused_across_seeds_and_sizes(gpu_available, pk_table):
+    """MT19937 on one instance across seeds and tracer counts (the jump
+    windows of each stream use are kept per seed and first draw, and built
+    ahead of use on a second stream): catalogues equal to fresh instances'.
+    50000 tracers need two 2^18-word segments of position and velocity
+    draws, 3000 one, so the windows are rebuilt in both directions."""
+    k, P = pk_table
+
+    def made(seed=None):
+        e = EZmock(Lbox=640., Ngrid=64, rng=1, seed=seed or 11, precision="f32")
+        e.eval_growth_params(0.5, 0.31)
+        e.setup_linear_pk(k, P)
+        return e
+
+    g = made()
+    for seed, ns in ((11, (3000, 50000, 3000)), (12, (50000,)), (11, (50000,))):
+        g.set_seed(seed)
+        g.create_dens_field_from_ic()
+        for n in ns:
+            a = g.populate_tracer(0.5, 8., 0.3, 150., n)
+            f = made(seed)
+            f.create_dens_field_from_ic()
+            b = f.populate_tracer(0.5, 8., 0.3, 150., n)
+            assert len(a[0]) == len(b[0]) > 0.9 * n
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (seed, n)
+
+
+def test_mt19937_reused_instance_mock512(gpu_available, pk_table):
+    """The same at 512^3 (Lbox 2500 Mpc/h, 3,125,000 tracers), where the
+    spectrum and bias draws need three-level jump trees built on two
+    streams at once: seed 42, then 77, then 42 again on one instance, each
+    equal to a fresh instance's noise field and catalogue."""
+    k, P = pk_table
+
+    def mock(e):
+        e.create_dens_field_from_ic()
+        e.populate_tracer_device(0.8, 8., 0.25, 200., 3125000)
+        return e.field("spec"), e.device_catalog()
+
+    def made(seed):
+        e = EZmock(Lbox=2500., Ngrid=512, rng=1, seed=seed, precision="f32")
+        e.eval_growth_params(0.5, 0.31)
+        e.setup_linear_pk(k, P)
+        e.keep_spectrum(True)
+        return e
+
+    g = made(42)
+    fresh = {}
+    for seed in (42, 77, 42):
+        g.set_seed(seed)
+        sg, cg = mock(g)
+        if seed not in fresh:
+            f = made(seed)
+            fresh[seed] = mock(f)
+            del f
+        sf, cf = fresh[seed]
+        assert np.array_equal(sg, sf), seed
+        assert len(cg[0]) == len(cf[0]), seed
+        for x, y in zip(cg, cf):
+            assert np.array_equal(x, y), seed
+
+
 def test_device_rsd_matches_reference_formula(gpu_available, pk_table):
     k, P = pk_table
     g = EZmock(Lbox=320., Ngrid=32, rng=0, seed=9, precision="f32")
