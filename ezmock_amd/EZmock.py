@@ -198,6 +198,26 @@ class EZmock:
         self._assert_return_value()
         return n.value
 
+    def mock(self, rho_c, rho_exp, pdf_base, sigma_v, ntracer, fixamp=False, iphase=False):
+        """create_dens_field_from_ic + populate_tracer_device in one call,
+        replayed from captured HIP graphs after the first; returns the
+        catalogue size (catalogue on the GPU)."""
+        params = np.array([rho_c, rho_exp, pdf_base, sigma_v], dtype=self._rt)
+        n = C.c_size_t(0)
+        self._err.value = self._L.EZmock_ext_mock(
+            self._ez, params.ctypes.data, int(ntracer), bool(fixamp), bool(iphase),
+            C.byref(n), C.byref(self._err))
+        self._assert_return_value()
+        return n.value
+
+    def path_counts(self):
+        """How this instance's calls ran: device-selection populates,
+        host-driven populates, capacity redos, graph replays, captures."""
+        o = np.zeros(5, dtype=np.uint64)
+        self._L.EZmock_ext_path_counts(self._ez, o.ctypes.data)
+        return dict(devsel=int(o[0]), host=int(o[1]), retry=int(o[2]), replay=int(o[3]),
+                    capture=int(o[4]))
+
     def device_catalog(self):
         """Copy the device-resident catalogue to host numpy arrays."""
         ptrs = (C.c_void_p * 6)()

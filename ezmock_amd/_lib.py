@@ -21,6 +21,7 @@ API_SYMBOLS = [
 ]
 EXT_SYMBOLS = [
     "EZmock_ext_real_size", "EZmock_ext_device_count", "EZmock_ext_populate_device",
+    "EZmock_ext_mock", "EZmock_ext_path_counts",
     "EZmock_ext_copy_tracers", "EZmock_ext_device_tracers", "EZmock_ext_rsd_z",
     "EZmock_ext_copy_field", "EZmock_ext_pdf_info", "EZmock_ext_set_option",
     "EZmock_ext_kernel_times", "EZmock_ext_reset_kernel_times", "EZmock_ext_sync",
@@ -72,6 +73,10 @@ def load(precision="f32"):
     L.EZmock_ext_device_count.restype = i32
     L.EZmock_ext_populate_device.restype = i32
     L.EZmock_ext_populate_device.argtypes = [vp, vp, sz, b, C.POINTER(sz), pint]
+    L.EZmock_ext_path_counts.restype = i32
+    L.EZmock_ext_path_counts.argtypes = [vp, vp]
+    L.EZmock_ext_mock.restype = i32
+    L.EZmock_ext_mock.argtypes = [vp, vp, sz, b, b, C.POINTER(sz), pint]
     L.EZmock_ext_copy_tracers.restype = i32
     L.EZmock_ext_copy_tracers.argtypes = [vp] + [vp] * 6 + [sz, pint]
     L.EZmock_ext_device_tracers.restype = i32

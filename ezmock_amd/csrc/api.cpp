@@ -14,6 +14,7 @@
 #include <atomic>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -137,6 +138,42 @@ struct Mesh {
   size_t nmax = 0, ncell = 0, ntot = 0, ncell0 = 0, ntot0 = 0;
   std::vector<size_t> pdf;
   unsigned long long hist_h[EZ_NUM_LOGB_BIN] = {0};
+  // populate with the selection on the device: control block, pinned
+  // staging (parameters in, results out), spectrum seed of a replayed mock
+  PopCtl *ctl_d = nullptr;
+  MrgState *seed_d = nullptr;
+  struct PopIO {
+    PopCtl in;
+    MrgState seed;
+    uint64_t num[EZ_MAX_NUM_TRACER_PER_CELL + 1];
+    PopCtl out;
+    uint64_t cnt[EZ_MAX_NUM_TRACER_PER_CELL + 1];
+    unsigned long long cflag;
+  } *pio = nullptr;
+  int devsel = 1;                       // 2: capacities of half the expected counts
+  // path counters: device-selection populates, host-driven ones, capacity
+  // retries, graph replays, graph captures
+  uint64_t npath[5] = {0, 0, 0, 0, 0};
+  // captured mock (EZmock_ext_mock): field and populate graphs, and what
+  // they were captured with
+  hipGraphExec_t g_field = nullptr, g_pop = nullptr;
+  bool use_graph = true;
+  struct GraphKey {
+    long pk_version = -1; double vfac = 0;
+    bool fixamp = false, iphase = false, keep_spec = false, fft_streams = false;
+    rkey *keys = nullptr; size_t keys_cap = 0; real *cat0 = nullptr; size_t cat_cap = 0;
+    uint32_t *cic_list = nullptr; size_t cic_list_cap = 0; uint32_t fcap = 0;
+    int cic_path = 0; bool cic_fixed_ok = false;
+    int sched[3] = {0, 0, 0};
+    bool operator==(const GraphKey &o) const {
+      return pk_version == o.pk_version && vfac == o.vfac && fixamp == o.fixamp &&
+          iphase == o.iphase && keep_spec == o.keep_spec && fft_streams == o.fft_streams &&
+          keys == o.keys && keys_cap == o.keys_cap && cat0 == o.cat0 && cat_cap == o.cat_cap &&
+          cic_list == o.cic_list && cic_list_cap == o.cic_list_cap && fcap == o.fcap &&
+          cic_path == o.cic_path && cic_fixed_ok == o.cic_fixed_ok &&
+          sched[0] == o.sched[0] && sched[1] == o.sched[1] && sched[2] == o.sched[2];
+    }
+  } gkey;
   // options
   bool keep_spec = false, profile = false;
   std::vector<Timer> pending;
@@ -144,6 +181,9 @@ struct Mesh {
 };
 
 std::atomic<long> g_pk_version{0};   // P(k) tables of all instances
+
+std::shared_mutex g_api_mu;
+thread_local int t_api_depth = 0;
 
 Conf *CONF(EZMOCK *ez) { return (Conf *) ez->conf; }
 RngConf *RNG(EZMOCK *ez) { return (RngConf *) ez->rng; }
@@ -214,8 +254,17 @@ void free_tracer_scratch(Mesh *m) {
   dfree(&m->sort_gh); dfree(&m->sort_go); dfree(&m->sort_scan);
 }
 
+void drop_graphs(Mesh *m) {
+  if (m->g_field) (void) hipGraphExecDestroy(m->g_field);
+  if (m->g_pop) (void) hipGraphExecDestroy(m->g_pop);
+  m->g_field = m->g_pop = nullptr;
+}
+
 void free_mesh_device(Mesh *m) {
   if (m->smt) (void) hipStreamSynchronize(m->smt);   // window builds ahead of use
+  drop_graphs(m);
+  dfree(&m->ctl_d); dfree(&m->seed_d);
+  if (m->pio) { (void) hipHostFree(m->pio); m->pio = nullptr; }
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
   dfree(&m->vel4); dfree(&m->gen_base);
   dfree(&m->rho); dfree(&m->ny); dfree(&m->slot0); dfree(&m->rhot); dfree(&m->nt);
@@ -838,6 +887,186 @@ int attach_particles(EZMOCK *ez, Mesh *m, size_t ntot, size_t *iv,
 
 // Catalogue arrays for the caller (EZmock_populate_tracer): malloc'd host
 // arrays filled from pinned staging buffers.
+int ensure_keys(Mesh *m, size_t cap) {
+  if (m->keys_cap >= cap) return 0;
+  free_tracer_scratch(m);
+  const size_t hw = sort_hist_words(cap);
+  int e;
+  if ((e = dalloc(&m->keys, cap)) || (e = dalloc(&m->keys_tmp, cap)) ||
+      (e = dalloc(&m->sort_gh, hw)) || (e = dalloc(&m->sort_go, hw)) ||
+      (e = dalloc(&m->sort_scan, scan_tmp_words(hw) + 8)))
+    return e;
+  m->keys_cap = cap;
+  return 0;
+}
+
+// internal: a device-selection populate outgrew its capacities (the
+// host-driven path then sizes them from the counts)
+constexpr int EZ_RETRY = -1;
+
+// Capacities of a device-selection populate.  Candidates (cells at or above
+// the threshold bin, a few times ncell <= ntot) and tracers (ntot and the
+// ties of the PDF walk) are bounded in terms of eval_pdf's ntot, which
+// follows the expected number of tracers and barely the bias parameters, so
+// that one allocation serves a sweep of parameters: the current buffers are
+// kept while they hold 2 ntot + 64 Ki keys (at most N) and ntot + ntot/32 +
+// 1024 tracers, else they grow to 3 ntot + 64 Ki and ntot + ntot/8 + 4096.
+// With alloc false: whether the current ones are kept.
+bool dsel_capacity(Mesh *m, size_t N, size_t ntot, bool alloc, int *e) {
+  size_t kacc = 2 * ntot + 65536, kcap = 3 * ntot + 65536;
+  if (kacc > N) kacc = N;
+  if (kcap > N) kcap = N;
+  const size_t cacc = ntot + ntot / 32 + 1024;
+  size_t ccap = ntot + ntot / 8 + 4096;
+  if (m->devsel == 2) {                 // capacities too small: the redo path
+    kacc = kcap = ntot / 4 + 1;
+    ccap = ntot / 2 + 1;
+  }
+  const bool keep = m->keys_cap >= kacc && m->cat_cap >= cacc && m->cat[0];
+  if (keep) return true;
+  if (!alloc) return false;
+  if (m->keys_cap < kacc && (*e = ensure_keys(m, kcap))) return false;
+  if (m->cat_cap < cacc || !m->cat[0]) {
+    for (int i = 0; i < 6; i++) dfree(&m->cat[i]);
+    m->cat_cap = 0;
+    for (int i = 0; i < 6; i++)
+      if ((*e = dalloc(&m->cat[i], ccap))) return false;
+    m->cat_cap = ccap;
+  }
+  return true;
+}
+
+// The control block of a device-selection populate into the pinned
+// staging, and its upload with the PDF bin sizes (stream order).  Every
+// call of the engine ends with a synchronisation, so the staging is free
+// again when the next call writes it.
+int dsel_upload(Mesh *m, const BiasParams &bp, const std::vector<size_t> &num, size_t ncell,
+    real sigma_v) {
+  int e;
+  if ((e = dalloc(&m->ctl_d, 1))) return e;
+  if (!m->pio && hipHostMalloc((void **) &m->pio, sizeof(Mesh::PopIO)) != hipSuccess) {
+    m->pio = nullptr;
+    return EZMOCK_ERR_MEMORY;
+  }
+  PopCtl &c = m->pio->in;
+  memset(&c, 0, sizeof c);
+  const uint32_t *jt = mrg_host_jump_table();
+  c.bp = bp;
+  mrg_jump(c.bp.seed_state, 2 * (uint64_t) bp.N, jt);
+  c.pos_seed = bp.seed_state;
+  mrg_jump(c.pos_seed, 5 * (uint64_t) bp.N, jt);
+  c.vel_seed = bp.seed_state;
+  mrg_jump(c.vel_seed, 8 * (uint64_t) bp.N, jt);
+  c.sigma_v = (double) sigma_v;
+  c.ncell = ncell;
+  c.kcap = m->keys_cap;
+  c.ccap = m->cat_cap;
+  c.nmax = (int) num.size();
+  for (size_t p = 0; p < num.size(); p++) m->pio->num[p] = num[p];
+  HIP_OK(hipMemcpyAsync(m->ctl_d, &c, sizeof c, hipMemcpyHostToDevice, m->st));
+  HIP_OK(hipMemcpyAsync(m->d_num, m->pio->num, sizeof(uint64_t) * num.size(),
+      hipMemcpyHostToDevice, m->st));
+  return 0;
+}
+
+// The launches of a device-selection populate, pop_tracer.c:149-1007 (and
+// with vel :593-753, :1043-1113): valid cells, bias model, threshold bin,
+// candidates, sort, PDF walk, tracers per cell, positions (velocities),
+// then the control block, the PDF bin counts and the compaction flag into
+// the pinned staging.  Every count is read from the control block, so the
+// sequence can be captured once and replayed for other seeds and
+// parameters.
+int dsel_launch(EZMOCK *ez, Mesh *m, bool vel) {
+  Conf *conf = CONF(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const size_t nblk = (N + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  const size_t nch = (nblk + EZ_CELLS_PER_BLOCK - 1) / EZ_CELLS_PER_BLOCK;
+  PopCtl *ctl = m->ctl_d;
+  BiasParams bp{};                      // the grid size only: the rest from ctl
+  bp.N = N;
+  const MrgState none{};
+  {
+    Scope s(m, "bias_count");
+    KCHK(launch_count_valid(m->rho, 0, N, m->blockcnt, m->st, ctl));
+  }
+  {
+    Scope s(m, "scan");
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+  }
+  KCHK(launch_zero(m->hist, EZ_NUM_LOGB_BIN * sizeof(unsigned long long), m->st));
+  {
+    Scope s(m, "bias");
+    KCHK(launch_bias(m->rho, m->rhot, m->blockoff, m->hist, m->blockhist, bp, m->jtab,
+        nullptr, m->st, ctl));
+  }
+  {
+    Scope s(m, "select");
+    KCHK(launch_select(m->hist, m->scan_tmp + nch, ctl, m->st));
+  }
+  KCHK(launch_zero(m->counter, 2 * sizeof(unsigned long long), m->st));
+  {
+    Scope s(m, "compact");
+    KCHK(launch_block_cand(m->blockhist, nblk, 0, m->blockcnt, m->st, ctl));
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+    KCHK(launch_compact_at(m->rhot, 0, N, m->blockoff, m->blockcnt, m->keys, m->keys_cap,
+        m->counter, m->st, ctl));
+  }
+  {
+    Scope s(m, "sort");
+    KCHK(launch_sort(m->keys, m->keys_tmp, m->keys_cap, m->sort_gh, m->sort_go, m->sort_scan,
+        m->st, &ctl->nsort));
+  }
+  {
+    Scope s(m, "walk");
+    KCHK(launch_walk(m->keys, m->keys_cap, m->d_num, 0, m->d_vb, m->d_cnt, m->d_res, m->st,
+        ctl, m->scan_tmp + nch));
+  }
+  {
+    Scope s(m, "assign_nt");
+    KCHK(launch_assign_nt(m->rhot, N, m->d_vb, 0, m->nt, m->blockcnt, m->st, ctl));
+  }
+  {
+    Scope s(m, "scan");
+    KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
+  }
+  const real gs = conf->Lbox / conf->Ng;
+  const real igs = conf->Ng / conf->Lbox;
+  {
+    Scope s(m, "positions");
+    KCHK(launch_positions(m->nt, m->blockoff, Ng, conf->Lbox, gs, none, m->jtab, nullptr,
+        m->cat[0], m->cat[1], m->cat[2], m->cat_cap, m->st, ctl));
+  }
+  if (vel) {
+    Scope s(m, "velocities");
+    KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, m->cat_cap, 0, Ng, igs,
+        COSMO(ez)->vfac, 0, none, m->jtab, nullptr, m->cat[3], m->cat[4], m->cat[5], m->st,
+        ctl));
+  }
+  HIP_OK(hipMemcpyAsync(&m->pio->out, ctl, sizeof(PopCtl), hipMemcpyDeviceToHost, m->st));
+  HIP_OK(hipMemcpyAsync(m->pio->cnt, m->d_cnt, sizeof(m->pio->cnt), hipMemcpyDeviceToHost,
+      m->st));
+  HIP_OK(hipMemcpyAsync(&m->pio->cflag, m->counter, sizeof(unsigned long long),
+      hipMemcpyDeviceToHost, m->st));
+  return 0;
+}
+
+// After the synchronisation that follows dsel_launch: the errors of the
+// host-driven path, in its order (EZ_RETRY where a capacity was short), and
+// the PDF of the instance.
+int dsel_collect(Mesh *m) {
+  const PopCtl &r = m->pio->out;
+  if (r.flags & POP_RHO_LESS) return EZMOCK_ERR_RHO_LESS;
+  if (r.flags & (POP_KEYS_FULL | POP_CAT_FULL)) return EZ_RETRY;
+  if ((r.flags & (POP_NO_BIN | POP_WALK_ERR | POP_CAND_ERR)) || m->pio->cflag)
+    return EZMOCK_ERR_UNKNOWN;
+  m->nmax = (size_t) r.nmax;
+  m->ncell = r.ncell_out;
+  m->ntot = r.ntot;
+  m->pdf.assign(m->pio->cnt, m->pio->cnt + r.nmax);
+  return 0;
+}
+
 struct HostCat {
   real *h[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   void release() {
@@ -869,7 +1098,7 @@ void copy_arrays(real *const *dst, real *const *src, int a0, int a1, size_t n, i
 
 // The device part of EZmock_populate_tracer; catalogue left in m->cat.
 int populate_device(EZMOCK *ez, const real *params, size_t nexp,
-    bool att_part, size_t *ntracer, HostCat *hc = nullptr) {
+    bool att_part, size_t *ntracer, HostCat *hc = nullptr, bool allow_dsel = true) {
   Conf *conf = CONF(ez);
   Mesh *m = MESH(ez);
   Cosmo *cosmo = COSMO(ez);
@@ -931,6 +1160,38 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   const real rho_max = 1 + lambda * 5;
   bp.imax = ilogb(rho_max);
   mrg_seed_state(RNG(ez)->seed, bp.seed_state);
+  const real gs = conf->Lbox / conf->Ng;
+  const real igs = conf->Ng / conf->Lbox;
+  size_t iv = 0;
+  // MT19937: the velocity draws' words on a second stream while the
+  // position draws' words and the positions are made (the jump trees are
+  // latency-bound: the two overlap)
+  const bool vel_side = RNG(ez)->type == 1 && m->fft_streams && !m->profile;
+  uint64_t nvel = 0;
+  const uint32_t *mtw_vel = nullptr;
+  // selection on the device (MRG32k3a, tracers at uniform positions, fields
+  // owned by the engine): no host round trip before the end, or before the
+  // velocities when the catalogue goes to host arrays
+  const bool dsel = allow_dsel && m->devsel && RNG(ez)->type == 0 && !att_part &&
+      !m->psi_ref && m->mom_valid;
+  if (dsel) m->npath[0]++;
+  else m->npath[1]++;
+  if (dsel) {
+    if (!dsel_capacity(m, N, ntot, true, &e)) return e;
+    if ((e = dsel_upload(m, bp, num, ncell, sigma_v)) || (e = dsel_launch(ez, m, !hc)))
+      return e;
+    if (hc) {
+      if ((e = sync_and_collect(m))) return e;
+      if ((e = dsel_collect(m)) == EZ_RETRY) {
+        m->npath[2]++;
+        return populate_device(ez, params, nexp, att_part, ntracer, hc, false);
+      }
+      if (e) return e;
+      ncell = m->ncell;
+      ntot = m->ntot;
+    }
+  }
+  else {
   {
     Scope s(m, "bias_count");
     KCHK(launch_count_valid(m->rho, thres, N, m->blockcnt, m->st));
@@ -968,16 +1229,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   if (imax == EZ_NUM_LOGB_BIN) return EZMOCK_ERR_UNKNOWN;
   const real rho_thres = (imax == EZ_NUM_LOGB_BIN - 1) ? (real) EZ_REAL_MIN
       : (real) pow(2, bp.imax - imax);
-  if (m->keys_cap < cnt) {
-    free_tracer_scratch(m);
-    const size_t cap = cnt + cnt / 4 + 4096;
-    const size_t hw = sort_hist_words(cap);
-    if ((e = dalloc(&m->keys, cap)) || (e = dalloc(&m->keys_tmp, cap)) ||
-        (e = dalloc(&m->sort_gh, hw)) || (e = dalloc(&m->sort_go, hw)) ||
-        (e = dalloc(&m->sort_scan, scan_tmp_words(hw) + 8)))
-      return e;
-    m->keys_cap = cap;
-  }
+  if (m->keys_cap < cnt && (e = ensure_keys(m, cnt + cnt / 4 + 4096))) return e;
   KCHK(launch_zero(m->counter, 2 * sizeof(unsigned long long), m->st));
   {
     Scope s(m, "compact");
@@ -1026,20 +1278,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     KCHK(launch_scan_u32(m->blockcnt, m->blockoff, nblk, m->scan_tmp, m->st));
   }
   if ((e = ensure_cat(m, ntot))) return e;
-  const real gs = conf->Lbox / conf->Ng;
-  const real igs = conf->Ng / conf->Lbox;
   // tracers attached to particles first (att_part), then the others at
   // uniform positions in their cells, pop_tracer.c:881-1007
-  size_t iv = 0;
   const int16_t *ntpos = m->nt;
   const uint64_t *offpos = m->blockoff;
   if (att_part && (e = attach_particles(ez, m, ntot, &iv, &ntpos, &offpos))) return e;
-  // MT19937: the velocity draws' words on a second stream while the
-  // position draws' words and the positions are made (the jump trees are
-  // latency-bound: the two overlap)
-  const bool vel_side = RNG(ez)->type == 1 && m->fft_streams && !m->profile;
-  const uint64_t nvel = 6 * (((uint64_t) ntot + 1) / 2);
-  const uint32_t *mtw_vel = nullptr;
+  nvel = 6 * (((uint64_t) ntot + 1) / 2);
   if (vel_side) {
     HIP_OK(hipEventRecord(m->ev_fork, m->st));
     HIP_OK(hipStreamWaitEvent(m->sx[1], m->ev_fork, 0));
@@ -1053,6 +1297,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     Scope s(m, "positions");
     KCHK(launch_positions(ntpos, offpos, Ng, conf->Lbox, gs, bp.seed_state,
         m->jtab, mtw_pos, m->cat[0] + iv, m->cat[1] + iv, m->cat[2] + iv, ntot - iv, m->st));
+  }
   }
   m->natt = iv;
   // host catalogue: the positions go to pinned staging on a second stream
@@ -1095,11 +1340,12 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
   }
   if (vel_side) HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[1], 0));
   else if ((e = mt_words(ez, m, 3, 8 * (uint64_t) N, nvel, &mtw_vel))) return e;
-  {
+  if (!dsel || hc) {                    // (launched with the selection otherwise)
     Scope s(m, "velocities");
     KCHK(launch_velocities(m->cat[0], m->cat[1], m->cat[2], m->vel4, ntot, iv, Ng, igs,
         cosmo->vfac, (double) sigma_v,
-        bp.seed_state, m->jtab, mtw_vel, m->cat[3], m->cat[4], m->cat[5], m->st));
+        bp.seed_state, m->jtab, mtw_vel, m->cat[3], m->cat[4], m->cat[5], m->st,
+        dsel ? m->ctl_d : nullptr));
   }
   if (hc) {
     for (int i = 3; i < 6; i++)
@@ -1114,6 +1360,14 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     return e;
   }
   if (hc) copy_arrays(hc->h, stage, 3, 6, ntot, conf->nthread);
+  if (dsel && !hc) {
+    if ((e = dsel_collect(m)) == EZ_RETRY) {
+      m->npath[2]++;
+      return populate_device(ez, params, nexp, att_part, ntracer, hc, false);
+    }
+    if (e) return e;
+    ntot = m->ntot;
+  }
   m->ntracer = ntot;
   *ntracer = ntot;
   Par *par = PAR(ez);
@@ -1137,6 +1391,318 @@ int populate_checks(EZMOCK *ez, const real *params, size_t nexp,
   if (!m->have_field) return EZMOCK_ERR_ARG_MESH;
   if (COSMO(ez)->vfac == HUGE_VAL) return EZMOCK_ERR_ARG_COSMO;
   return 0;
+}
+
+// The displacement fields of the initial conditions (delta null) or of a
+// white-noise field (delta, host): spectrum, packed kz = 0 / Nyquist slots,
+// first FFT pass of the three components, then their last two passes (on
+// three streams unless disabled or timed), dens_field.c:337-432 and
+// perturb.c:78-399.  seedp (optional): the spectrum's MRG32k3a seed in
+// device memory, for a captured sequence.  Launches only, apart from the
+// first field of a grid size, which times the FFT schedules.
+int field_launch(EZMOCK *ez, Mesh *m, real *delta, bool fixamp, bool iphase,
+    const MrgState *seedp) {
+  Conf *conf = CONF(ez);
+  PkTab *pk = PK(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  const int Ngh = Ng / 2;
+  const size_t plane = (size_t) Ng * Ng;
+  int e;
+  GenParams gp;
+  gp.Ng = Ng;
+  gp.Nh = plane * (Ngh + 1);
+  gp.kfac = EZ_PI * 2 / conf->Lbox;
+  gp.logkfac = log(gp.kfac);
+  gp.Pfac = pow(conf->Lbox, -1.5);
+  gp.logpk = pk->log;
+  gp.fixamp = fixamp;
+  gp.iphase = iphase;
+  mrg_seed_state(RNG(ez)->seed, gp.seed_state);
+  real *spec = nullptr;
+  if (m->keep_spec) {
+    if ((e = dalloc(&m->spec, 2 * gp.Nh))) return e;
+    spec = m->spec;
+  }
+  cplx *rhok = (cplx *) m->rho;
+  if (delta) {
+    HIP_OK(hipMemcpyAsync(m->psi[0], delta, N * sizeof(real),
+        hipMemcpyHostToDevice, m->st));
+    if ((e = r2c_field(m, Ng, m->psi[0]))) return e;
+    Scope s(m, "gen_spectrum_wn");
+    KCHK(launch_gen_rhok_wn(rhok, m->ny + plane, (const cplx *) m->psi[0],
+        m->ny, gp, m->ptab, spec, m->st));
+  }
+  else {
+    const uint32_t *mtw = nullptr;
+    if ((e = mt_words(ez, m, 0, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw)))
+      return e;
+    // the bias draws' windows (at most N + 1 draws from 2N) on their own
+    // stream while the field is made
+    if (RNG(ez)->type == 1 && (e = mt_windows(ez, m, 1, 2 * (uint64_t) N, (uint64_t) N + 1,
+        m->profile ? m->st : m->smt)))
+      return e;
+    if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return e;
+    Scope s(m, "gen_spectrum");
+    KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
+        m->st, seedp));
+  }
+  // the generated Nyquist plane lives in ny[plane, 2 plane)
+  cplx *ny_save = m->ny;
+  m->ny = m->ny + plane;
+  if ((e = dalloc(&m->slot0, 3 * plane))) { m->ny = ny_save; return e; }
+  {
+    Scope s(m, "fft_slot0");
+    if (fft_slot0_pack(rhok, m->ny, m->slot0, Ng, m->st) != hipSuccess) {
+      m->ny = ny_save;
+      return EZMOCK_ERR_UNKNOWN;
+    }
+  }
+  // the first field of a grid size (>= 256) times the FFT schedule
+  // candidates on these buffers and keeps the fastest for the process
+  bool tuned = false;
+  (void) fft_schedule(Ng, &tuned);
+  if (m->fft_tune && Ng >= 256 && !tuned &&
+      fft_autotune(Ng, (cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2], rhok,
+          m->ny, m->slot0, m->twN, m->st, nullptr, nullptr) != hipSuccess) {
+    m->ny = ny_save;
+    return EZMOCK_ERR_UNKNOWN;
+  }
+  {
+    Scope s(m, "fft_pass_i");
+    if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
+        rhok, m->ny, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
+      m->ny = ny_save;
+      return EZMOCK_ERR_UNKNOWN;
+    }
+  }
+  // the three components' remaining passes are independent: components
+  // 1 and 2 on their own streams, forked after the first pass and joined
+  // before the CIC
+  // (one stream while per-kernel timers are on, so that their intervals
+  // do not overlap)
+  const bool fork = m->fft_streams && !m->profile;
+  hipStream_t cs[3] = {m->st, m->st, m->st};
+  if (fork) {
+    HIP_OK(hipEventRecord(m->ev_fork, m->st));
+    for (int i = 0; i < 2; i++) {
+      HIP_OK(hipStreamWaitEvent(m->sx[i], m->ev_fork, 0));
+      cs[1 + i] = m->sx[i];
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    if ((e = c2r_finish(m, Ng, m->psi[c], cs[c]))) { m->ny = ny_save; return e; }
+  }
+  m->ny = ny_save;
+  if (fork) {
+    for (int i = 0; i < 2; i++) {
+      HIP_OK(hipEventRecord(m->ev_join[i], m->sx[i]));
+      HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[i], 0));
+    }
+  }
+  if (delta) {
+    const real norm = 1 / sqrt((double) N);
+    Scope s(m, "scale");
+    for (int c = 0; c < 3; c++) KCHK(launch_scale(m->psi[c], norm, N, m->st));
+  }
+  return 0;
+}
+
+// What a captured mock depends on besides its seed and parameters (which
+// it reads from device memory): buffers that can be reallocated, the P(k)
+// table, the velocity factor, the spectrum options and the FFT schedule.
+Mesh::GraphKey graph_key(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
+  Mesh::GraphKey k;
+  k.pk_version = PK(ez) ? PK(ez)->version : -1;
+  k.vfac = COSMO(ez)->vfac;
+  k.fixamp = fixamp;
+  k.iphase = iphase;
+  k.keep_spec = m->keep_spec;
+  k.fft_streams = m->fft_streams;
+  k.keys = m->keys;
+  k.keys_cap = m->keys_cap;
+  k.cat0 = m->cat[0];
+  k.cat_cap = m->cat_cap;
+  k.cic_list = m->cic_list;
+  k.cic_list_cap = m->cic_list_cap;
+  k.fcap = m->cic_fcap;
+  k.cic_path = m->cic_path;
+  k.cic_fixed_ok = m->cic_fixed_ok;
+  bool tuned = false;
+  const FftSchedule fs = fft_schedule(CONF(ez)->Ng, &tuned);
+  k.sched[0] = fs.tk_j;
+  k.sched[1] = fs.tk_i;
+  k.sched[2] = fs.split;
+  return k;
+}
+
+// A mock's launches (the field from the initial conditions with its CIC,
+// then the populate with the selection on the device) captured as two HIP
+// graphs, right after a mock by the plain calls has sized every buffer.
+// Nothing is captured where a replay could not reproduce the plain calls
+// without a host round trip: MT19937 streams, timers on, fields by
+// reference, CIC by tile scan or exact ranges, FFT schedule not yet tuned.
+void capture_mock(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
+  drop_graphs(m);
+  Conf *conf = CONF(ez);
+  const int Ng = conf->Ng;
+  bool tuned = false;
+  (void) fft_schedule(Ng, &tuned);
+  if (!m->use_graph || m->profile || m->devsel != 1 || RNG(ez)->type != 0 || m->psi_ref ||
+      !m->mom_valid || m->cic_path != 0 || !m->cic_fixed_ok || !m->cic_list ||
+      (m->fft_tune && Ng >= 256 && !tuned) || !m->pio || !m->ctl_d || !m->cic_ovf_h)
+    return;
+  if (dalloc(&m->seed_d, 1)) return;
+  hipGraphExec_t x[2] = {nullptr, nullptr};
+  bool ok = true;
+  for (int i = 0; i < 2 && ok; i++) {
+    if (hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      ok = false;
+      break;
+    }
+    int e = 0;
+    if (i == 0) {
+      e = field_launch(ez, m, nullptr, fixamp, iphase, m->seed_d);
+      if (!e) e = paint_cic(m, Ng, (real) (conf->Ng / conf->Lbox), m->rho, true);
+    }
+    else e = dsel_launch(ez, m, true);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(m->st, &g);
+    if (e || ce != hipSuccess || !g ||
+        hipGraphInstantiate(&x[i], g, nullptr, nullptr, 0) != hipSuccess) {
+      x[i] = nullptr;
+      ok = false;
+    }
+    if (g) (void) hipGraphDestroy(g);
+  }
+  m->cic_pend.kind = 0;                 // the capture's painting is not pending
+  (void) hipGetLastError();
+  if (!ok) {
+    for (auto &g : x) if (g) (void) hipGraphExecDestroy(g);
+    fprintf(stderr, "EZmock: the mock could not be captured as a HIP graph; "
+        "plain launches are used\n");
+    m->use_graph = false;
+    return;
+  }
+  m->g_field = x[0];
+  m->g_pop = x[1];
+  m->gkey = graph_key(ez, m, fixamp, iphase);
+  m->npath[4]++;
+}
+
+}  // namespace
+
+ApiGuard::ApiGuard() : held(t_api_depth++ == 0) {
+  if (held) g_api_mu.lock_shared();
+}
+
+ApiGuard::~ApiGuard() {
+  --t_api_depth;
+  if (held) g_api_mu.unlock_shared();
+}
+
+namespace {
+
+// EZmock_ext_mock under the API guard: a replay of the instance's graphs,
+// or the plain calls (with *recapture set when they succeeded, for a
+// capture after the guard is released).
+int mock_call(EZMOCK *ez, const real *params, const size_t nexp, const bool fixamp,
+    const bool iphase, size_t *ntracer, int *err, bool *recapture) {
+  ApiGuard guard;
+  Mesh *m = MESH(ez);
+  Conf *conf = CONF(ez);
+  const int Ng = conf->Ng;
+  const size_t N = (size_t) Ng * Ng * Ng;
+  // the plain calls, then a capture for the next mock, unless the graphs
+  // of an earlier mock still describe this instance
+  auto plain = [&](bool *cap) {
+    EZmock_create_dens_field(ez, nullptr, false, nullptr, fixamp, iphase, err);
+    EZmock_ext_populate_device(ez, params, nexp, false, ntracer, err);
+    *cap = *err == EZMOCK_SUCCESS;
+    return *err;
+  };
+  if (!m->g_field || !m->use_graph || m->profile || m->devsel != 1 || RNG(ez)->type != 0 ||
+      !PK(ez) || !(m->gkey == graph_key(ez, m, fixamp, iphase)) ||
+      m->ptab_version != PK(ez)->version)
+    return plain(recapture);
+  // the checks of the two calls, in their order
+  const double kmin = 2 * EZ_PI / conf->Lbox;
+  const double kmax = EZ_PI * conf->Ng / conf->Lbox * sqrt(3);
+  PkTab *pk = PK(ez);
+  if (pk->kmin > kmin - EZ_DOUBLE_TOL || pk->kmax < kmax + EZ_DOUBLE_TOL)
+    return (*err = EZMOCK_ERR_PK_RANGE);
+  int e = populate_checks(ez, params, nexp, ntracer);
+  if (e) return (*err = e);
+  const real rho_c = params[0], rho_exp = params[1], pbase = params[2], sigma_v = params[3];
+  if (!isfinite(rho_c) || rho_c < 0) return (*err = EZMOCK_ERR_PAR_RCUT);
+  if (!isfinite(rho_exp) || rho_exp <= 0) return (*err = EZMOCK_ERR_PAR_REXP);
+  if (!isfinite(pbase) || pbase <= 0 || pbase >= 1) return (*err = EZMOCK_ERR_PAR_PBASE);
+  if (!isfinite(sigma_v) || sigma_v < 0) return (*err = EZMOCK_ERR_PAR_SIGV);
+  std::vector<size_t> num;
+  size_t ncell, ntot;
+  if ((e = eval_pdf(nexp, pbase, N, num, ncell, ntot))) return (*err = e);
+  if (!dsel_capacity(m, N, ntot, false, &e)) return plain(recapture);
+  m->ncell0 = ncell;
+  m->ntot0 = ntot;
+  BiasParams bp;
+  bp.N = N;
+  bp.thres = (rho_c == 0) ? (real) EZ_REAL_MIN : rho_c;
+  bp.rho_sat = (real) EZ_RHO_SAT;
+  bp.iexp = -1 / rho_exp;
+  const real lambda = (real) EZ_GAUSS_WIDTH;
+  bp.lambda = lambda;
+  const real rho_max = 1 + lambda * 5;
+  bp.imax = ilogb(rho_max);
+  mrg_seed_state(RNG(ez)->seed, bp.seed_state);
+  m->pio->seed = bp.seed_state;
+  m->have_field = false;
+  m->psi_ref = false;
+  for (int i = 0; i < 3; i++) m->psi_host[i] = nullptr;
+  HIP_OK(hipMemcpyAsync(m->seed_d, &m->pio->seed, sizeof(MrgState), hipMemcpyHostToDevice,
+      m->st));
+  if ((e = dsel_upload(m, bp, num, ncell, sigma_v))) return (*err = e);
+  m->npath[3]++;
+  HIP_OK(hipGraphLaunch(m->g_field, m->st));
+  HIP_OK(hipGraphLaunch(m->g_pop, m->st));
+  if ((e = sync_and_collect(m))) return (*err = e);
+  if (*m->cic_ovf_h) {
+    // a CIC list range overflowed: the plain calls repaint with exact
+    // ranges and resize the ranges of later fields
+    drop_graphs(m);
+    return plain(recapture);
+  }
+  m->fixamp = fixamp;
+  m->iphase = iphase;
+  m->mom_valid = true;
+  m->have_field = true;
+  if ((e = dsel_collect(m)) == EZ_RETRY) {
+    m->npath[2]++;
+    drop_graphs(m);
+    e = populate_device(ez, params, nexp, false, ntracer);
+    *recapture = !e;
+    return (*err = e);
+  }
+  if (e) return (*err = e);
+  m->natt = 0;
+  m->ntracer = m->ntot;
+  *ntracer = m->ntot;
+  Par *par = PAR(ez);
+  par->rho_c = rho_c;
+  par->rho_exp = rho_exp;
+  par->pdf_base = pbase;
+  par->sigma_v = sigma_v;
+  return EZMOCK_SUCCESS;
+}
+
+
+// capture_mock with every other thread's entry into the C ABI held off;
+// skipped when called from inside another entry point of this thread
+void capture_exclusive(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
+  if (t_api_depth != 0) return;
+  std::unique_lock<std::shared_mutex> lk(g_api_mu);
+  t_api_depth++;
+  capture_mock(ez, m, fixamp, iphase);
+  t_api_depth--;
 }
 
 }  // namespace
@@ -1180,6 +1746,7 @@ EZMOCK *EZmock_init(const double Lbox, const int Ngrid, const int randgen,
 
 // EZmock.c:133-142
 void EZmock_destroy(EZMOCK *ez) {
+  ApiGuard guard;
   if (!ez) return;
   Mesh *m = MESH(ez);
   if (m) { free_mesh_device(m); delete m; }
@@ -1338,6 +1905,7 @@ int EZmock_setup_linear_pk(EZMOCK *ez, const double *k, const int n,
 // dens_field.c:337-432
 int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
     real *delta, const bool fixamp, const bool iphase, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
@@ -1405,102 +1973,7 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
       dfree(&dk); dfree(&dP); dfree(&dP2);
       m->ptab_version = pk->version;
     }
-    GenParams gp;
-    gp.Ng = Ng;
-    gp.Nh = plane * (Ngh + 1);
-    gp.kfac = EZ_PI * 2 / conf->Lbox;
-    gp.logkfac = log(gp.kfac);
-    gp.Pfac = pow(conf->Lbox, -1.5);
-    gp.logpk = pk->log;
-    gp.fixamp = fixamp;
-    gp.iphase = iphase;
-    mrg_seed_state(RNG(ez)->seed, gp.seed_state);
-    real *spec = nullptr;
-    if (m->keep_spec) {
-      if ((e = dalloc(&m->spec, 2 * gp.Nh))) return (*err = e);
-      spec = m->spec;
-    }
-    cplx *rhok = (cplx *) m->rho;
-    if (delta) {
-      HIP_OK(hipMemcpyAsync(m->psi[0], delta, N * sizeof(real),
-          hipMemcpyHostToDevice, m->st));
-      if ((e = r2c_field(m, Ng, m->psi[0]))) return (*err = e);
-      Scope s(m, "gen_spectrum_wn");
-      KCHK(launch_gen_rhok_wn(rhok, m->ny + plane, (const cplx *) m->psi[0],
-          m->ny, gp, m->ptab, spec, m->st));
-    }
-    else {
-      const uint32_t *mtw = nullptr;
-      if ((e = mt_words(ez, m, 0, 0, (fixamp ? 1 : 2) * (gp.Nh - 1), &mtw)))
-        return (*err = e);
-      // the bias draws' windows (at most N + 1 draws from 2N) on their own
-      // stream while the field is made
-      if (RNG(ez)->type == 1 && (e = mt_windows(ez, m, 1, 2 * (uint64_t) N, (uint64_t) N + 1,
-          m->profile ? m->st : m->smt)))
-        return (*err = e);
-      if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return (*err = e);
-      Scope s(m, "gen_spectrum");
-      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
-          m->st));
-    }
-    // the generated Nyquist plane lives in ny[plane, 2 plane)
-    cplx *ny_save = m->ny;
-    m->ny = m->ny + plane;
-    if ((e = dalloc(&m->slot0, 3 * plane))) { m->ny = ny_save; return (*err = e); }
-    {
-      Scope s(m, "fft_slot0");
-      if (fft_slot0_pack(rhok, m->ny, m->slot0, Ng, m->st) != hipSuccess) {
-        m->ny = ny_save;
-        return (*err = EZMOCK_ERR_UNKNOWN);
-      }
-    }
-    // the first field of a grid size (>= 256) times the FFT schedule
-    // candidates on these buffers and keeps the fastest for the process
-    bool tuned = false;
-    (void) fft_schedule(Ng, &tuned);
-    if (m->fft_tune && Ng >= 256 && !tuned &&
-        fft_autotune(Ng, (cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2], rhok,
-            m->ny, m->slot0, m->twN, m->st, nullptr, nullptr) != hipSuccess) {
-      m->ny = ny_save;
-      return (*err = EZMOCK_ERR_UNKNOWN);
-    }
-    {
-      Scope s(m, "fft_pass_i");
-      if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
-          rhok, m->ny, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
-        m->ny = ny_save;
-        return (*err = EZMOCK_ERR_UNKNOWN);
-      }
-    }
-    // the three components' remaining passes are independent: components
-    // 1 and 2 on their own streams, forked after the first pass and joined
-    // before the CIC
-    // (one stream while per-kernel timers are on, so that their intervals
-    // do not overlap)
-    const bool fork = m->fft_streams && !m->profile;
-    hipStream_t cs[3] = {m->st, m->st, m->st};
-    if (fork) {
-      HIP_OK(hipEventRecord(m->ev_fork, m->st));
-      for (int i = 0; i < 2; i++) {
-        HIP_OK(hipStreamWaitEvent(m->sx[i], m->ev_fork, 0));
-        cs[1 + i] = m->sx[i];
-      }
-    }
-    for (int c = 0; c < 3; c++) {
-      if ((e = c2r_finish(m, Ng, m->psi[c], cs[c]))) { m->ny = ny_save; return (*err = e); }
-    }
-    m->ny = ny_save;
-    if (fork) {
-      for (int i = 0; i < 2; i++) {
-        HIP_OK(hipEventRecord(m->ev_join[i], m->sx[i]));
-        HIP_OK(hipStreamWaitEvent(m->st, m->ev_join[i], 0));
-      }
-    }
-    if (delta) {
-      const real norm = 1 / sqrt((double) N);
-      Scope s(m, "scale");
-      for (int c = 0; c < 3; c++) KCHK(launch_scale(m->psi[c], norm, N, m->st));
-    }
+    if ((e = field_launch(ez, m, delta, fixamp, iphase, nullptr))) return (*err = e);
   }
   // density and (for fields owned by the engine) the momentum fields used
   // by every later populate call
@@ -1518,6 +1991,7 @@ int EZmock_create_dens_field(EZMOCK *ez, real *psi[3], bool deepcopy,
 int EZmock_populate_tracer(EZMOCK *ez, const real *params, const size_t nexp,
     const bool att_part, size_t *ntracer, real **x, real **y, real **z,
     real **vx, real **vy, real **vz, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   int e = populate_checks(ez, params, nexp, ntracer);
@@ -1613,6 +2087,7 @@ int EZmock_write_ascii(EZMOCK *ez, const real *x, const real *y, const real *z,
 int EZmock_ext_real_size(void) { return (int) sizeof(real); }
 
 int EZmock_ext_device_count(void) {
+  ApiGuard guard;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
@@ -1620,6 +2095,7 @@ int EZmock_ext_device_count(void) {
 
 int EZmock_ext_populate_device(EZMOCK *ez, const real *params,
     const size_t nexp, const bool att_part, size_t *ntracer, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   int e = populate_checks(ez, params, nexp, ntracer);
@@ -1628,8 +2104,21 @@ int EZmock_ext_populate_device(EZMOCK *ez, const real *params,
   return EZMOCK_SUCCESS;
 }
 
+int EZmock_ext_mock(EZMOCK *ez, const real *params, const size_t nexp, const bool fixamp,
+    const bool iphase, size_t *ntracer, int *err) {
+  if (!err) return EZMOCK_ERR_ARG_ECODE;
+  if (*err != EZMOCK_SUCCESS) return *err;
+  if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
+  Mesh *m = MESH(ez);
+  bool recapture = false;
+  const int r = mock_call(ez, params, nexp, fixamp, iphase, ntracer, err, &recapture);
+  if (recapture) capture_exclusive(ez, m, fixamp, iphase);
+  return r;
+}
+
 int EZmock_ext_copy_tracers(EZMOCK *ez, real *x, real *y, real *z, real *vx,
     real *vy, real *vz, const size_t n, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
@@ -1647,6 +2136,7 @@ int EZmock_ext_copy_tracers(EZMOCK *ez, real *x, real *y, real *z, real *vx,
 
 int EZmock_ext_device_tracers(EZMOCK *ez, void *ptrs[6], size_t *ntracer,
     int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
@@ -1657,6 +2147,7 @@ int EZmock_ext_device_tracers(EZMOCK *ez, void *ptrs[6], size_t *ntracer,
 }
 
 int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
@@ -1684,6 +2175,7 @@ int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd, int *err) 
 int EZmock_ext_powspec(EZMOCK *ez, const double rsd_fac, const bool interlace,
     const double dk, const int nbin, double *k, double *P0, double *P2,
     double *nmodes, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
@@ -1751,6 +2243,7 @@ int EZmock_ext_powspec(EZMOCK *ez, const double rsd_fac, const bool interlace,
 }
 
 int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err != EZMOCK_SUCCESS) return *err;
   if (!ez || !dst) return (*err = EZMOCK_ERR_ARG_NULL);
@@ -1777,6 +2270,12 @@ int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
   return EZMOCK_SUCCESS;
 }
 
+int EZmock_ext_path_counts(EZMOCK *ez, uint64_t *out) {
+  if (!ez || !out) return EZMOCK_ERR_ARG_NULL;
+  for (int i = 0; i < 5; i++) out[i] = MESH(ez)->npath[i];
+  return EZMOCK_SUCCESS;
+}
+
 int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n) {
   if (!ez || !out || n < 5) return EZMOCK_ERR_ARG_NULL;
   Mesh *m = MESH(ez);
@@ -1787,6 +2286,7 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n) {
 }
 
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
+  ApiGuard guard;
   if (!ez) return EZMOCK_ERR_ARG_EZ;
   Mesh *m = MESH(ez);
   if (option == 1) m->keep_spec = value != 0;
@@ -1797,6 +2297,11 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
   }
   else if (option == 4) m->fft_streams = value != 0;
   else if (option == 6) m->fft_tune = value != 0;
+  else if (option == 7 && value >= 0 && value <= 2) m->devsel = (int) value;
+  else if (option == 8) {
+    m->use_graph = value != 0;
+    if (!m->use_graph) drop_graphs(m);
+  }
   else if (option == 5 && value >= 0) {
     m->cic_list_limit = (size_t) value;
     m->cic_fixed_ok = true;
@@ -1864,6 +2369,7 @@ int EZmock_ext_set_seed(EZMOCK *ez, const uint64_t seed, int *err) {
 }
 
 int EZmock_ext_sync(EZMOCK *ez, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (!ez) return (*err = EZMOCK_ERR_ARG_EZ);
   Mesh *m = MESH(ez);
@@ -1875,6 +2381,7 @@ int EZmock_ext_sync(EZMOCK *ez, int *err) {
 int EZmock_ext_uniform(const int randgen, const uint64_t seed,
     const uint64_t start, const size_t n, const bool pos, double *out,
     int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err) return *err;
   if (randgen != 0 && randgen != 1) return (*err = EZMOCK_ERR_ARG_RNG);
@@ -1989,6 +2496,7 @@ int EZmock_ext_fft_schedule(const int Ng, int *out) {
 
 int EZmock_ext_fft_set_schedule(const int Ng, const int tk_j, const int tk_i,
     const int split) {
+  ApiGuard guard;
   if (!fft_supported(Ng)) return EZMOCK_ERR_ARG_NGRID;
   auto ok = [](int t) { return t == 0 || t == 4 || t == 8 || t == 16; };
   if (!ok(tk_j) || !ok(tk_i) || (split != 0 && split != 1)) return EZMOCK_ERR_ARG_PARAMS;
@@ -2000,6 +2508,7 @@ int EZmock_ext_fft_set_schedule(const int Ng, const int tk_j, const int tk_i,
 // device against their closed form (no host copies of the fields).
 int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const int *ijk,
     const double *amp, double *maxerr, double *maxref, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err) return *err;
   if (!fft_supported(Ng)) return (*err = EZMOCK_ERR_ARG_NGRID);
@@ -2088,16 +2597,19 @@ int EZmock_ext_fft_check(const int Ng, const int path, const int nmodes, const i
 }
 
 int EZmock_ext_fft_c2r(const int Ng, const real *in, real *out, int *err) {
+  ApiGuard guard;
   return fft_test(Ng, in, out, +1, err);
 }
 
 int EZmock_ext_fft_r2c(const int Ng, const real *in, real *out, int *err) {
+  ApiGuard guard;
   return fft_test(Ng, in, out, -1, err);
 }
 
 int EZmock_ext_pdf_walk(const real *rhot, const size_t num, const size_t ncell,
     const size_t *pdf, const int nmax, int16_t *nt, size_t *pdf_out,
     size_t *ncell_out, size_t *ntot_out, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err) return *err;
   if (!rhot || !pdf || !nt || nmax < 1 || nmax > EZ_MAX_NUM_TRACER_PER_CELL)

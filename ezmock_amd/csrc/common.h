@@ -60,6 +60,16 @@ typedef uint64_t rkey;
 //   tracer t position            -> 5N + 3t   pop_tracer.c:919
 //   velocity scatter, pair q     -> 8N + 6q   pop_tracer.c:1071
 
+// Every entry point of the C ABI that calls HIP holds this guard (shared,
+// once per thread however deeply the entry points nest); the capture of a
+// mock's HIP graphs holds the same lock exclusively, since HIP refuses work
+// on the legacy stream, from any thread, while a stream is being captured.
+struct ApiGuard {
+  bool held;
+  ApiGuard();
+  ~ApiGuard();
+};
+
 #define HIP_OK(call)                                                        \
   do {                                                                      \
     hipError_t e_ = (call);                                                 \

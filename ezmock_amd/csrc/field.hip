@@ -131,11 +131,12 @@ __device__ __forceinline__ void gen_mirror(const GenCell &c, int i, int j, int k
 // per workgroup, so that no workgroup waits on a serial jump of its own.
 __global__ void __launch_bounds__(256)
 k_gen_base(MrgState seed, uint64_t mult, size_t nblk, size_t cells_per_blk,
-    const uint32_t *__restrict__ jtab, MrgState *__restrict__ base) {
+    const uint32_t *__restrict__ jtab, MrgState *__restrict__ base,
+    const MrgState *__restrict__ seedp) {
   const size_t b = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const size_t blk0 = b * cells_per_blk;
-  MrgState g = seed;
+  MrgState g = seedp ? *seedp : seed;
   mrg_jump(g, (blk0 == 0 ? 0 : blk0 - 1) * mult, jtab);
   base[b] = g;
 }
@@ -241,7 +242,7 @@ size_t gen_base_count(size_t Nh) {
 
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    const uint32_t *mtw, MrgState *bases, hipStream_t st) {
+    const uint32_t *mtw, MrgState *bases, hipStream_t st, const MrgState *seedp) {
   if (gp.Ng & 1) return hipErrorInvalidValue;
   constexpr int CH = GEN_CHUNK;
   const size_t nthr = (gp.Nh + CH - 1) / CH;
@@ -253,7 +254,7 @@ hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     if (!bases) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gen_base, dim3(ez_blocks(nblk, 256)), dim3(256), 0, st,
         gp.seed_state, (uint64_t) (gp.fixamp ? 1 : 2), nblk, (size_t) GEN_TPB * CH, jtab,
-        bases);
+        bases, seedp);
     hipLaunchKernelGGL((k_gen_rhok<CH>), dim3(nblk), dim3(GEN_TPB), 0,
         st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, (const MrgState *) bases);
   }

@@ -19,9 +19,11 @@ hipError_t launch_pk_table(double *tab, int nsq, const double *kt,
 // bases: gen_base_count(gp.Nh) states of scratch (MRG32k3a; unused with
 // MT19937 words).
 size_t gen_base_count(size_t Nh);
+// seedp (optional): the MRG32k3a seed state in device memory, in place of
+// gp.seed_state.
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
-    const uint32_t *mtw, MrgState *bases, hipStream_t st);
+    const uint32_t *mtw, MrgState *bases, hipStream_t st, const MrgState *seedp = nullptr);
 hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     const cplx *dk_ny, const GenParams &gp, const double *ptab,
     real *ref_copy, hipStream_t st);
@@ -103,44 +105,82 @@ struct BiasParams {
 // Cells per block of the per-cell passes (256 threads x 16 cells).
 #define EZ_CELLS_PER_BLOCK 4096
 
+// Control block of a populate with the selection on the device (no host
+// round trip between the bias model and the catalogue), in device memory.
+// The host writes the parameters before the launch, the kernels the
+// results; kernels given a null control block use their own arguments.
+struct PopCtl {
+  BiasParams bp;                // seed_state advanced to draw 2N
+  MrgState pos_seed, vel_seed;  // advanced to draws 5N and 8N
+  double sigma_v;
+  uint64_t ncell;               // cells eval_pdf asks for
+  uint64_t kcap, ccap;          // capacities: candidate keys, catalogue
+  int nmax;
+  // written by the kernels
+  int imax;                     // bin where the cumulative histogram reaches ncell
+  real rho_thres;
+  uint64_t nvalid, cnt, nsort, ncell_out, ntot;
+  uint32_t flags;               // POP_* bits
+};
+#define POP_RHO_LESS 1u         // fewer valid cells than ncell
+#define POP_NO_BIN 2u           // the histogram never reaches ncell
+#define POP_KEYS_FULL 4u        // more candidates than kcap
+#define POP_CAT_FULL 8u         // more tracers than ccap
+#define POP_WALK_ERR 16u        // the PDF walk ran past the candidates
+#define POP_CAND_ERR 32u        // candidate counts disagree
+
 hipError_t launch_count_valid(const real *rho, real thres, size_t N,
-    uint32_t *blockcnt, hipStream_t st);
+    uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl = nullptr);
 hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
     uint64_t *tmp, hipStream_t st);
 size_t scan_tmp_words(size_t n);
 // blockhist (optional): per-block 64-bin histograms (nblk x 64 u32).
 // mtw (MT19937, else nullptr): stream words of draws 2N + 2 p, pair p.
+// ctl: bp (its seed already at draw 2N) from the control block.
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
     unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
-    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st);
+    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st, const PopCtl *ctl = nullptr);
+// The threshold bin, candidate count and error flags of the control block
+// from the bias histogram and the valid-cell count *nvalid.
+hipError_t launch_select(const unsigned long long *hist, const uint64_t *nvalid, PopCtl *ctl,
+    hipStream_t st);
 hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
-    uint32_t *blockcnt, hipStream_t st);
+    uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl = nullptr);
 hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
     const uint64_t *blockoff, const uint32_t *blockcnt, rkey *keys, size_t cap,
-    unsigned long long *flag, hipStream_t st);
+    unsigned long long *flag, hipStream_t st, const PopCtl *ctl = nullptr);
 hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
     size_t cap, unsigned long long *counter, hipStream_t st);
 // 64-bin log2 histogram of nonzero tracer densities, added to hist.
 hipError_t launch_hist_rhot(const real *rhot, size_t N, unsigned long long *hist,
     hipStream_t st);
+// np (optional): the number of keys in device memory, n then being the
+// capacity (<= n keys sorted, grids sized by n).
 hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
-    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st);
+    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st, const uint64_t *np = nullptr);
 size_t sort_hist_words(size_t n);
+// ctl: M and nmax from the control block, results and flags into it
+// (ncand: the candidate count of the compaction's scan).
 hipError_t launch_walk(const rkey *sorted, size_t M, const uint64_t *num,
-    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st);
+    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st,
+    PopCtl *ctl = nullptr, const uint64_t *ncand = nullptr);
 hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
-    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st);
+    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st, const PopCtl *ctl = nullptr);
 // mtw (MT19937, else nullptr): stream words of draws 5N + 3 t, tracer t.
+// ctl: the seed from the control block (ntot: the catalogue capacity).
 hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
-    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st);
+    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st,
+    const PopCtl *ctl = nullptr);
 // mtw (MT19937, else nullptr): stream words of draws 8N + 6 q, pair q.
 // Tracers t < natt (attached to particles) keep the velocity already in
 // vx/vy/vz; the others take the CIC gather; all get the sigma_v scatter.
+// ctl: seed, sigma_v and the tracer count from the control block (ntot:
+// the catalogue capacity, which sizes the grid).
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
     const real4 *vel, size_t ntot, size_t natt, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
-    real *vz, hipStream_t st);
+    real *vz, hipStream_t st, const PopCtl *ctl = nullptr);
 // tracers attached to particles (pop_tracer.c:517-582, 804-879)
 struct AttGeom {
   int Ng;

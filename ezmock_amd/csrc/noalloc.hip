@@ -87,6 +87,7 @@ extern "C" size_t EZmock_ext_noalloc_scratch_bytes(void) { return NA_SCRATCH_BYT
 extern "C" int EZmock_ext_pdf_walk_noalloc(const real *rhot_dev, const size_t num,
     const size_t ncell, const size_t *pdf, const int nmax, int16_t *nt_dev,
     size_t *pdf_out, size_t *ncell_out, size_t *ntot_out, void *stream, int *err) {
+  ApiGuard guard;
   if (!err) return EZMOCK_ERR_ARG_ECODE;
   if (*err) return *err;
   if (!rhot_dev || !nt_dev || !pdf || !pdf_out || !ncell_out || !ntot_out || num == 0 ||

@@ -17,8 +17,9 @@
 //     after evicting least recently used idle instances of other shapes if
 //     that makes room;
 //   * every request is computed by the same calls as a stand-alone mock
-//     (EZmock_set_cosmology, EZmock_setup_linear_pk,
-//     EZmock_create_dens_field, EZmock_ext_populate_device and optionally
+//     (EZmock_set_cosmology, EZmock_setup_linear_pk, EZmock_ext_mock, i.e.
+//     EZmock_create_dens_field + EZmock_ext_populate_device replayed from
+//     the instance's captured HIP graphs, and optionally
 //     EZmock_ext_powspec), so its result does not depend on what else runs.
 // Written against the public C API only.
 #include <algorithm>
@@ -110,9 +111,8 @@ int run_one(EZMOCK_POOL *p, Inst &in, const Job &job, EZMOCK_RESULT &r) {
   int err = 0;
   if (!in.ez && (err = init_inst(p, in, q.seed))) return err;
   EZmock_ext_set_seed(in.ez, q.seed, &err);
-  EZmock_create_dens_field(in.ez, nullptr, false, nullptr, false, false, &err);
   size_t ntr = 0;
-  EZmock_ext_populate_device(in.ez, q.params, q.nexp, false, &ntr, &err);
+  EZmock_ext_mock(in.ez, q.params, q.nexp, false, false, &ntr, &err);
   if (err) return err;
   r.ntracer = ntr;
   if (q.pk_nbin > 0) {

@@ -117,8 +117,9 @@ hipError_t launch_scan_u32(const uint32_t *in, uint64_t *out, size_t n,
 // ------------------------------------------------------------ bias model
 __global__ void __launch_bounds__(TPB)
 k_count_valid(const real *__restrict__ rho, real thres, size_t N,
-    uint32_t *__restrict__ blockcnt) {
+    uint32_t *__restrict__ blockcnt, const PopCtl *__restrict__ ctl) {
   __shared__ uint32_t ws[TPB / 64];
+  if (ctl) thres = ctl->bp.thres;
   const size_t base = (size_t) blockIdx.x * CPB;
   // unconditional loads at clamped indices: all in flight together (a
   // guarded load waits for its data at the branch join)
@@ -140,9 +141,9 @@ k_count_valid(const real *__restrict__ rho, real thres, size_t N,
 }
 
 hipError_t launch_count_valid(const real *rho, real thres, size_t N,
-    uint32_t *blockcnt, hipStream_t st) {
+    uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl) {
   hipLaunchKernelGGL(k_count_valid, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      rho, thres, N, blockcnt);
+      rho, thres, N, blockcnt, ctl);
   return hipGetLastError();
 }
 
@@ -185,8 +186,9 @@ __device__ __forceinline__ void gauss_pair(Draws &s, double sigma,
 __global__ void __launch_bounds__(TPB)
 k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
     const uint64_t *__restrict__ blockoff, unsigned long long *__restrict__ hist,
-    uint32_t *__restrict__ blockhist, BiasParams bp, const uint32_t *__restrict__ jtab,
-    const uint32_t *__restrict__ mtw) {
+    uint32_t *__restrict__ blockhist, BiasParams bp_in, const uint32_t *__restrict__ jtab,
+    const uint32_t *__restrict__ mtw, const PopCtl *__restrict__ ctl) {
+  const BiasParams bp = ctl ? ctl->bp : bp_in;
   __shared__ real tile[PADIDX(CPB)];
   __shared__ uint16_t vcell[CPB];       // block-local rank -> cell
   __shared__ uint32_t lh[EZ_NUM_LOGB_BIN];
@@ -280,11 +282,48 @@ k_bias(const real *__restrict__ rho, real *__restrict__ rhot,
 
 hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
     unsigned long long *hist, uint32_t *blockhist, const BiasParams &bp,
-    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st) {
+    const uint32_t *jtab, const uint32_t *mtw, hipStream_t st, const PopCtl *ctl) {
   BiasParams b2 = bp;
-  mrg_jump(b2.seed_state, (uint64_t) 2 * bp.N, mrg_host_jump_table());
+  if (!ctl) mrg_jump(b2.seed_state, (uint64_t) 2 * bp.N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_bias, dim3(ez_blocks(bp.N, CPB)), dim3(TPB), 0, st,
-      rho, rhot, blockoff, hist, blockhist, b2, jtab, mtw);
+      rho, rhot, blockoff, hist, blockhist, b2, jtab, mtw, ctl);
+  return hipGetLastError();
+}
+
+// pop_tracer.c:361-369 on the device: the first log2 bin where the
+// cumulative count of tracer densities reaches ncell, its threshold
+// 2^(imax_bias - imax) (REAL_MIN for the last bin) and the candidates to
+// sort; one thread (64 bins).
+__global__ void __launch_bounds__(64)
+k_select(const unsigned long long *__restrict__ hist, const uint64_t *__restrict__ nvalid_p,
+    PopCtl *__restrict__ ctl) {
+  if (threadIdx.x) return;
+  const uint64_t nvalid = *nvalid_p, ncell = ctl->ncell;
+  uint32_t fl = 0;
+  if (nvalid < ncell) fl |= POP_RHO_LESS;
+  uint64_t c = 0;
+  int im;
+  for (im = 0; im < EZ_NUM_LOGB_BIN; im++) {
+    c += hist[im];
+    if (c >= ncell) break;
+  }
+  if (im == EZ_NUM_LOGB_BIN) {
+    fl |= POP_NO_BIN;
+    im = EZ_NUM_LOGB_BIN - 1;
+  }
+  ctl->imax = im;
+  ctl->rho_thres = (im == EZ_NUM_LOGB_BIN - 1) ? (real) EZ_REAL_MIN
+      : (real) ldexp(1.0, ctl->bp.imax - im);
+  ctl->nvalid = nvalid;
+  ctl->cnt = c;
+  ctl->nsort = c > ctl->kcap ? ctl->kcap : c;
+  if (c > ctl->kcap) fl |= POP_KEYS_FULL;
+  ctl->flags = fl;
+}
+
+hipError_t launch_select(const unsigned long long *hist, const uint64_t *nvalid, PopCtl *ctl,
+    hipStream_t st) {
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, st, hist, nvalid, ctl);
   return hipGetLastError();
 }
 
@@ -386,18 +425,19 @@ hipError_t launch_compact(const real *rhot, real thres, size_t N, rkey *keys,
 // log2 bins <= imax, i.e. rhot >= 2^(imax_bias - imax)); their exclusive
 // scan gives every block its output range.
 __global__ void k_block_cand(const uint32_t *__restrict__ blockhist, size_t nblk,
-    int imax, uint32_t *__restrict__ blockcnt) {
+    int imax, uint32_t *__restrict__ blockcnt, const PopCtl *__restrict__ ctl) {
   const size_t b = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
+  if (ctl) imax = ctl->imax;
   uint32_t c = 0;
   for (int i = 0; i <= imax; i++) c += blockhist[b * EZ_NUM_LOGB_BIN + i];
   blockcnt[b] = c;
 }
 
 hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
-    uint32_t *blockcnt, hipStream_t st) {
+    uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl) {
   hipLaunchKernelGGL(k_block_cand, dim3(ez_blocks(nblk, 256)), dim3(256), 0, st,
-      blockhist, nblk, imax, blockcnt);
+      blockhist, nblk, imax, blockcnt, ctl);
   return hipGetLastError();
 }
 
@@ -405,8 +445,10 @@ hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
 __global__ void __launch_bounds__(TPB)
 k_compact_at(const real *__restrict__ rhot, real thres, size_t N,
     const uint64_t *__restrict__ blockoff, const uint32_t *__restrict__ blockcnt,
-    rkey *__restrict__ keys, size_t cap, unsigned long long *__restrict__ flag) {
+    rkey *__restrict__ keys, size_t cap, unsigned long long *__restrict__ flag,
+    const PopCtl *__restrict__ ctl) {
   __shared__ uint32_t ws[TPB / 64];
+  if (ctl) thres = ctl->rho_thres;
   const size_t b0 = (size_t) blockIdx.x * CPB;
   real v[CPT];
   uint32_t c = 0;
@@ -434,9 +476,9 @@ k_compact_at(const real *__restrict__ rhot, real thres, size_t N,
 
 hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
     const uint64_t *blockoff, const uint32_t *blockcnt, rkey *keys, size_t cap,
-    unsigned long long *flag, hipStream_t st) {
+    unsigned long long *flag, hipStream_t st, const PopCtl *ctl) {
   hipLaunchKernelGGL(k_compact_at, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      rhot, thres, N, blockoff, blockcnt, keys, cap, flag);
+      rhot, thres, N, blockoff, blockcnt, keys, cap, flag, ctl);
   return hipGetLastError();
 }
 
@@ -447,11 +489,16 @@ hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
 
 __global__ void __launch_bounds__(TPB)
 k_sort_hist(const rkey *__restrict__ in, size_t n, int shift,
-    uint32_t *__restrict__ gh, size_t nblk) {
+    uint32_t *__restrict__ gh, size_t nblk, const uint64_t *__restrict__ np) {
   __shared__ uint32_t h[256];
+  if (np) n = *np;
+  const size_t base = (size_t) blockIdx.x * CPB;
+  if (base >= n) {                        // past the keys: an empty column
+    gh[(size_t) threadIdx.x * nblk + blockIdx.x] = 0;
+    return;
+  }
   h[threadIdx.x] = 0;
   __syncthreads();
-  const size_t base = (size_t) blockIdx.x * CPB;
   rkey kv[SORT_ITEMS];
 #pragma unroll
   for (int it = 0; it < SORT_ITEMS; it++) {
@@ -469,11 +516,14 @@ k_sort_hist(const rkey *__restrict__ in, size_t n, int shift,
 
 __global__ void __launch_bounds__(TPB)
 k_sort_scatter(const rkey *__restrict__ in, rkey *__restrict__ out, size_t n,
-    int shift, const uint64_t *__restrict__ gofs, size_t nblk) {
+    int shift, const uint64_t *__restrict__ gofs, size_t nblk,
+    const uint64_t *__restrict__ np) {
   __shared__ uint64_t bofs[256];
   __shared__ uint32_t wcnt[TPB / 64][256];
   __shared__ uint32_t wofs[TPB / 64][256];
   __shared__ uint32_t run[256];
+  if (np) n = *np;
+  if ((size_t) blockIdx.x * CPB >= n) return;     // uniform over the block
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   bofs[tid] = gofs[(size_t) tid * nblk + blockIdx.x];
   run[tid] = 0;
@@ -526,7 +576,7 @@ size_t sort_hist_words(size_t n) { return 256 * ((n + CPB - 1) / CPB); }
 // sort_hist_words(n) u32; gofs the same number of u64; scan_tmp
 // scan_tmp_words(sort_hist_words(n)) u64.
 hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
-    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st) {
+    uint64_t *gofs, uint64_t *scan_tmp, hipStream_t st, const uint64_t *np) {
   if (n == 0) return hipSuccess;
   const size_t nblk = (n + CPB - 1) / CPB;
   rkey *src = keys, *dst = tmp;
@@ -534,11 +584,11 @@ hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
   for (int p = 0; p < npass; p++) {
     const int shift = 8 * p;
     hipLaunchKernelGGL(k_sort_hist, dim3((unsigned) nblk), dim3(TPB), 0, st,
-        src, n, shift, ghist, nblk);
+        src, n, shift, ghist, nblk, np);
     hipError_t e = launch_scan_u32(ghist, gofs, 256 * nblk, scan_tmp, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned) nblk), dim3(TPB), 0, st,
-        src, dst, n, shift, gofs, nblk);
+        src, dst, n, shift, gofs, nblk, np);
     rkey *t = src; src = dst; dst = t;
   }
   // npass is even (4 or 8): the result is back in `keys`.
@@ -554,8 +604,13 @@ hipError_t launch_sort(rkey *keys, rkey *tmp, size_t n, uint32_t *ghist,
 __global__ void __launch_bounds__(64)
 k_walk(const rkey *__restrict__ sorted, size_t M, const uint64_t *__restrict__ num,
     int nmax, rkey *__restrict__ vb, uint64_t *__restrict__ cnt,
-    uint64_t *__restrict__ res) {
+    uint64_t *__restrict__ res, PopCtl *__restrict__ ctl,
+    const uint64_t *__restrict__ ncand) {
   const int lane = threadIdx.x;
+  if (ctl) {
+    M = ctl->nsort;
+    nmax = ctl->nmax;
+  }
   size_t s = 0;
   uint64_t ntot = 0;
   int err = 0;
@@ -582,22 +637,36 @@ k_walk(const rkey *__restrict__ sorted, size_t M, const uint64_t *__restrict__ n
     last = v;
     s = pos;
   }
-  if (lane == 0) { res[0] = s; res[1] = ntot; res[2] = err; }
+  if (lane == 0) {
+    res[0] = s; res[1] = ntot; res[2] = err;
+    if (ctl) {
+      uint32_t fl = ctl->flags;
+      if (err) fl |= POP_WALK_ERR;
+      if (ntot > ctl->ccap) fl |= POP_CAT_FULL;
+      if (ncand && *ncand != ctl->cnt) fl |= POP_CAND_ERR;
+      ctl->ncell_out = s;
+      ctl->ntot = ntot;
+      ctl->flags = fl;
+    }
+  }
 }
 
 hipError_t launch_walk(const rkey *sorted, size_t M, const uint64_t *num,
-    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st) {
+    int nmax, rkey *vb, uint64_t *cnt, uint64_t *res, hipStream_t st,
+    PopCtl *ctl, const uint64_t *ncand) {
   hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, st, sorted, M, num, nmax,
-      vb, cnt, res);
+      vb, cnt, res, ctl, ncand);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------ tracers per cell
 __global__ void __launch_bounds__(TPB)
 k_assign_nt(const real *__restrict__ rhot, size_t N, const rkey *__restrict__ vb,
-    int nmax, int16_t *__restrict__ nt, uint32_t *__restrict__ blocktr) {
+    int nmax, int16_t *__restrict__ nt, uint32_t *__restrict__ blocktr,
+    const PopCtl *__restrict__ ctl) {
   __shared__ rkey sb[EZ_MAX_NUM_TRACER_PER_CELL];
   __shared__ uint32_t ws[TPB / 64];
+  if (ctl) nmax = ctl->nmax;
   for (int p = threadIdx.x; p < nmax; p += TPB) sb[p] = vb[p];
   __syncthreads();
   const rkey vlast = sb[nmax - 1];
@@ -635,9 +704,9 @@ k_assign_nt(const real *__restrict__ rhot, size_t N, const rkey *__restrict__ vb
 }
 
 hipError_t launch_assign_nt(const real *rhot, size_t N, const rkey *vb,
-    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st) {
+    int nmax, int16_t *nt, uint32_t *blocktr, hipStream_t st, const PopCtl *ctl) {
   hipLaunchKernelGGL(k_assign_nt, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      rhot, N, vb, nmax, nt, blocktr);
+      rhot, N, vb, nmax, nt, blocktr, ctl);
   return hipGetLastError();
 }
 
@@ -650,7 +719,8 @@ __global__ void __launch_bounds__(TPB)
 k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockofs,
     FastDiv fNg, double Lbox, real gs, size_t N, MrgState seed,
     const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
-    real *__restrict__ x, real *__restrict__ y, real *__restrict__ z, size_t ntot) {
+    real *__restrict__ x, real *__restrict__ y, real *__restrict__ z, size_t ntot,
+    const PopCtl *__restrict__ ctl) {
   __shared__ int16_t tile[PADIDX(CPB)];
   __shared__ uint32_t toff[TPB];
   __shared__ uint64_t ws[TPB / 64];
@@ -674,7 +744,7 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
   toff[tid] = (uint32_t) excl;          // tot <= CPB * 1024
   const uint64_t tb = blockofs[blockIdx.x];
   if (tid == 0 && !mtw) {
-    MrgState g = seed;
+    MrgState g = ctl ? ctl->pos_seed : seed;
     mrg_jump(g, 3 * tb, jtab);
     base = g;
   }
@@ -730,13 +800,15 @@ k_positions(const int16_t *__restrict__ nt, const uint64_t *__restrict__ blockof
 
 hipError_t launch_positions(const int16_t *nt, const uint64_t *blockofs,
     int Ng, double Lbox, real gs, MrgState seed, const uint32_t *jtab,
-    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st) {
+    const uint32_t *mtw, real *x, real *y, real *z, size_t ntot, hipStream_t st,
+    const PopCtl *ctl) {
   const size_t N = (size_t) Ng * Ng * Ng;
   if (N >= 0xffffffffull) return hipErrorInvalidValue;
   MrgState s5 = seed;
-  mrg_jump(s5, (uint64_t) 5 * N, mrg_host_jump_table());
+  if (!ctl) mrg_jump(s5, (uint64_t) 5 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_positions, dim3(ez_blocks(N, CPB)), dim3(TPB), 0, st,
-      nt, blockofs, fastdiv_make((uint32_t) Ng), Lbox, gs, N, s5, jtab, mtw, x, y, z, ntot);
+      nt, blockofs, fastdiv_make((uint32_t) Ng), Lbox, gs, N, s5, jtab, mtw, x, y, z, ntot,
+      ctl);
   return hipGetLastError();
 }
 
@@ -809,8 +881,13 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     int Ng, real igs, double vfac, double sigma, uint64_t base_draw, MrgState seed,
     const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
     real *__restrict__ vx,
-    real *__restrict__ vy, real *__restrict__ vz) {
+    real *__restrict__ vy, real *__restrict__ vz, const PopCtl *__restrict__ ctl) {
   __shared__ MrgState base;
+  if (ctl) {                            // ntot: the catalogue capacity
+    if (ctl->ntot < ntot) ntot = ctl->ntot;
+    seed = ctl->vel_seed;
+    sigma = ctl->sigma_v;
+  }
   const size_t npair = (ntot + 1) / 2;
   const size_t q0blk = (size_t) xcd_item(blockIdx.x, gridDim.x) * TPB * PPT;
   if (threadIdx.x == 0 && !mtw) {
@@ -864,15 +941,15 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
 hipError_t launch_velocities(const real *x, const real *y, const real *z,
     const real4 *vel, size_t ntot, size_t natt, int Ng, real igs, double vfac, double sigma_v,
     MrgState seed, const uint32_t *jtab, const uint32_t *mtw, real *vx, real *vy,
-    real *vz, hipStream_t st) {
+    real *vz, hipStream_t st, const PopCtl *ctl) {
   const size_t N = (size_t) Ng * Ng * Ng;
   const size_t npair = (ntot + 1) / 2;
   if (npair == 0) return hipSuccess;
   MrgState s8 = seed;                       // draw 8N
-  mrg_jump(s8, (uint64_t) 8 * N, mrg_host_jump_table());
+  if (!ctl) mrg_jump(s8, (uint64_t) 8 * N, mrg_host_jump_table());
   hipLaunchKernelGGL(k_velocities, dim3(ez_blocks(npair, TPB * PPT)), dim3(TPB),
       0, st, x, y, z, vel, ntot, natt, Ng, igs, vfac, sigma_v,
-      (uint64_t) 0, s8, jtab, mtw, vx, vy, vz);
+      (uint64_t) 0, s8, jtab, mtw, vx, vy, vz, ctl);
   return hipGetLastError();
 }
 

@@ -24,6 +24,24 @@ int EZmock_ext_device_count(void);
 int EZmock_ext_populate_device(EZMOCK *ez, const real *params,
     const size_t nexp, const bool att_part, size_t *ntracer, int *err);
 
+/* One mock: EZmock_create_dens_field from the initial conditions
+   (fixamp, iphase) followed by EZmock_ext_populate_device without attached
+   particles, with the same results.  The first call runs those two calls
+   and captures their launches as two HIP graphs (field with its CIC; the
+   populate with the selection on the device), which later calls replay
+   for any seed (EZmock_ext_set_seed) and bias parameters, with one host
+   synchronisation per mock and no allocation.  The graphs are recaptured
+   after anything they depend on changes (P(k) table, cosmology, options,
+   a buffer that had to grow) and are not used with MT19937 streams, the
+   per-kernel timers, CIC by tile scan or exact ranges, or option 7 or 8
+   off; the plain calls then run instead.  A CIC list range or a capacity
+   that overflows during a replay makes the call redo that stage by the
+   plain calls.  The reference's counterpart is the pair of calls
+   (dens_field.c:337-432, pop_tracer.c:1140-1219), as the Python loop of a
+   calibration sweep makes them. */
+int EZmock_ext_mock(EZMOCK *ez, const real *params, const size_t nexp, const bool fixamp,
+    const bool iphase, size_t *ntracer, int *err);
+
 /* Copy the device-resident catalogue of the last populate call into six
    caller-owned host arrays of at least `n` elements (n <= ntracer). */
 int EZmock_ext_copy_tracers(EZMOCK *ez, real *x, real *y, real *z, real *vx,
@@ -48,6 +66,12 @@ int EZmock_ext_rsd_z(EZMOCK *ez, const double rsd_fac, double *z_rsd,
             reference layout (kept only if option 1 is set)
      7      log2 histogram               64 uint64                        */
 int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err);
+
+/* How the instance's calls ran, counted since EZmock_init: out[0] =
+   populates with the selection on the device, [1] = host-driven populates,
+   [2] = device-selection populates redone host-driven (capacity), [3] =
+   mocks replayed from captured graphs, [4] = graph captures. */
+int EZmock_ext_path_counts(EZMOCK *ez, uint64_t *out5);
 
 /* PDF of the last populate call: out[0]=nmax, [1]=ncell, [2]=ntot,
    [3]=ncell before ties, [4]=ntot before ties, [5..5+nmax) cells per bin. */
@@ -75,7 +99,22 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
             6 = FFT autotuning (1, default): the first field of a grid size
                 >= 256 in the process times the schedule candidates
                 (EZmock_ext_fft_schedule) and keeps the fastest; 0 = use the
-                grid's current schedule. */
+                grid's current schedule;
+            7 = selection on the device (1, default; 2 = with capacities of
+                a fraction of the counts, which exercises the redo): with
+                MRG32k3a and
+                tracers at uniform positions, the valid-cell count, the
+                threshold bin, the candidate count and the PDF walk's
+                results stay on the device (no host round trip before the
+                end of populate, or before the velocities with host
+                arrays); candidate keys and catalogue buffers are kept while
+                they hold 2 ntot + 64 Ki keys and ntot + ntot/32 + 1024
+                tracers (ntot: eval_pdf's count before ties), else grown to
+                3 ntot + 64 Ki and ntot + ntot/8 + 4096, and a populate that
+                outgrows them is redone by the host-driven path (0), which
+                sizes them from the counts.  Both give the same catalogue;
+            8 = HIP-graph replay in EZmock_ext_mock (1, default; 0 drops
+                the instance's graphs). */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
