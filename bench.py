@@ -123,11 +123,39 @@ def run_large(Ng=1536, L=3000.0):
             "alg_GBps": gbs, "peak_GBps": HBM_PEAK_GBS}
 
 
+def run_sweep(par, Ng, L, graph):
+    """The sweep on one instance, one mock after another through
+    EZmock_ext_mock: replayed from its captured HIP graphs (graph) or by the
+    plain create + populate calls (option 8 off)."""
+    from ezmock_amd import EZmock, inputs
+    k, P = inputs.eh_nowiggle_pk()
+    ez = EZmock(Lbox=L, Ngrid=Ng, rng=0, seed=1, precision="f32")
+    if not graph:
+        ez.set_option(8, 0)
+    ez.eval_growth_params(0.5, 0.31)
+    ez.setup_linear_pk(k, P)
+    for i in range(2):                                      # warm-up (and capture)
+        ez.set_seed(1000 + i)
+        ez.mock(*par[i], 300000)
+    ez.sync()
+    t0 = time.perf_counter()
+    for i, p in enumerate(par):
+        ez.set_seed(i + 1)
+        ez.mock(*p, 300000)
+    ez.sync()
+    dt = time.perf_counter() - t0
+    c = ez.path_counts()
+    del ez
+    return len(par) / dt, c
+
+
 def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
     """Calibration sweep: nmock mocks at Ng^3 with seeds 1..nmock and bias
     parameters drawn from a fixed meta-seed (rho_c in [0.2,1.2], rho_exp in
     [5,20], pdf_base in [0.1,0.5], sigma_v in [0,300]), 3e5 tracers each,
-    through the request pool (all submitted at once)."""
+    through the request pool (all submitted at once; plain launches, and
+    again with every worker's instance replaying its captured HIP graphs),
+    and on one instance with and without the graphs."""
     import numpy as np
     from ezmock_amd import inputs
     from ezmock_amd.pool import MockPool
@@ -136,8 +164,8 @@ def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
     par = [(rs.uniform(0.2, 1.2), rs.uniform(5, 20), rs.uniform(0.1, 0.5), rs.uniform(0, 300))
            for _ in range(nmock)]
     pool = MockPool(k, P, 0.5, 0.31, nslot=nslot, mem_budget=200e9)
-    for i in range(2):                                      # warm the workers
-        pool.wait(pool.submit(Ng, L, 1000 + i, par[i], 300000))
+    for w in [pool.submit(Ng, L, 1000 + i, par[i], 300000) for i in range(nslot)]:
+        pool.wait(w)                                        # warm the workers
     t0 = time.perf_counter()
     ids = [pool.submit(Ng, L, i + 1, par[i], 300000) for i in range(nmock)]
     res = [pool.wait(i) for i in ids]
@@ -149,8 +177,23 @@ def run_batch(nmock=64, Ng=256, L=1000.0, nslot=4):
     d1, _, _, _ = run_gpu(Ng, L, 1, par[0], 3e-4, "f32", 8, 2, profile=False)
     _, _, kt1, _ = run_gpu(Ng, L, 1, par[0], 3e-4, "f32", 8, 1, profile=True)
     kms = sum(t for t, c in kt1.values()) / 8
+    rg, cg = run_sweep(par, Ng, L, True)
+    rp, _ = run_sweep(par, Ng, L, False)
+    pool = MockPool(k, P, 0.5, 0.31, nslot=nslot, mem_budget=200e9, options={8: 1})
+    for w in [pool.submit(Ng, L, 1000 + i, par[i], 300000) for i in range(nslot)]:
+        pool.wait(w)
+    t0 = time.perf_counter()
+    ids = [pool.submit(Ng, L, i + 1, par[i], 300000) for i in range(nmock)]
+    resg = [pool.wait(i) for i in ids]
+    dtg = time.perf_counter() - t0
+    pool.close()
+    bad += [r["errmsg"] for r in resg if r["err"]]
     return {"mocks": nmock, "grid": Ng, "slots": nslot, "seconds": round(dt, 3),
             "mocks_per_s": round(nmock / dt, 2), "errors": bad[:3],
+            "pool_graph_mocks_per_s": round(nmock / dtg, 2),
+            "single_instance_graph_mocks_per_s": round(rg, 1),
+            "single_instance_plain_mocks_per_s": round(rp, 1),
+            "single_instance_graph_replays": cg["replay"],
             "single_instance_ms_per_mock": round(d1 * 1e3 / 8, 3),
             "kernel_ms_per_mock": round(kms, 3),
             "kernel_bound_mocks_per_s": round(1e3 / kms, 1)}

@@ -212,11 +212,12 @@ class EZmock:
 
     def path_counts(self):
         """How this instance's calls ran: device-selection populates,
-        host-driven populates, capacity redos, graph replays, captures."""
-        o = np.zeros(5, dtype=np.uint64)
+        host-driven populates, capacity redos, graph replays, captures, and
+        the candidate cells of the last device selection."""
+        o = np.zeros(6, dtype=np.uint64)
         self._L.EZmock_ext_path_counts(self._ez, o.ctypes.data)
         return dict(devsel=int(o[0]), host=int(o[1]), retry=int(o[2]), replay=int(o[3]),
-                    capture=int(o[4]))
+                    capture=int(o[4]), candidates=int(o[5]))
 
     def device_catalog(self):
         """Copy the device-resident catalogue to host numpy arrays."""

@@ -29,6 +29,7 @@ EXT_SYMBOLS = [
     "EZmock_ext_pdf_walk", "EZmock_ext_info", "EZmock_ext_pk_table",
     "EZmock_ext_powspec", "EZmock_ext_set_seed", "EZmock_ext_footprint",
     "EZmock_ext_pool_init", "EZmock_ext_pool_submit", "EZmock_ext_pool_wait",
+    "EZmock_ext_pool_option",
     "EZmock_ext_result_free", "EZmock_ext_pool_destroy", "EZmock_ext_fft_check",
     "EZmock_ext_fft_schedule", "EZmock_ext_fft_set_schedule",
     "EZmock_ext_noalloc_scratch_bytes", "EZmock_ext_pdf_walk_noalloc",
@@ -122,6 +123,8 @@ def load(precision="f32"):
     L.EZmock_ext_footprint.argtypes = [i32, i32]
     L.EZmock_ext_pool_init.restype = vp
     L.EZmock_ext_pool_init.argtypes = [i32, sz, vp, i32, vp, dbl, dbl, dbl, dbl, dbl, pint]
+    L.EZmock_ext_pool_option.restype = i32
+    L.EZmock_ext_pool_option.argtypes = [vp, i32, C.c_long]
     L.EZmock_ext_pool_submit.restype = C.c_long
     L.EZmock_ext_pool_submit.argtypes = [vp, vp, pint]
     L.EZmock_ext_pool_wait.restype = i32

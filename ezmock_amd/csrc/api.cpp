@@ -115,6 +115,9 @@ struct Mesh {
     uint64_t seed = 0, first = ~0ull;
     hipEvent_t ready = nullptr, used = nullptr;   // built; last read
   } mtwin[5];
+  // (created at the first MT19937 window build: HIP spreads streams over a
+  // few hardware queues in the order they are created, and one more stream
+  // per instance would put every instance's main stream on one queue)
   hipStream_t smt = nullptr;
   // tracers attached to particles: particles per cell, particle classes,
   // tracers left per cell, block counts / offsets, contested cells and
@@ -154,6 +157,7 @@ struct Mesh {
   // path counters: device-selection populates, host-driven ones, capacity
   // retries, graph replays, graph captures
   uint64_t npath[5] = {0, 0, 0, 0, 0};
+  uint64_t last_cnt = 0;                // candidates of the last device selection
   // captured mock (EZmock_ext_mock): field and populate graphs, and what
   // they were captured with
   hipGraphExec_t g_field = nullptr, g_pop = nullptr;
@@ -211,7 +215,6 @@ int device_ready(Mesh *m) {
     HIP_OK(hipStreamCreateWithFlags(&m->sx[i], hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&m->ev_join[i], hipEventDisableTiming));
   }
-  HIP_OK(hipStreamCreateWithFlags(&m->smt, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&m->ev_pos, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&m->ev_xyz, hipEventDisableTiming));
@@ -618,6 +621,17 @@ int r2c_field(Mesh *m, int Ng, real *field) {
   return 0;
 }
 
+// The stream that builds MT19937 windows ahead of use (the instance stream
+// while the per-kernel timers are on), created at its first use.
+hipStream_t mt_ahead_stream(Mesh *m) {
+  if (m->profile) return m->st;
+  if (!m->smt && hipStreamCreateWithFlags(&m->smt, hipStreamNonBlocking) != hipSuccess) {
+    m->smt = nullptr;
+    return m->st;
+  }
+  return m->smt;
+}
+
 // MT19937 segment windows of draws [first, first + n) for stream use k,
 // kept from an earlier build when seed and first match and it covers n
 // draws; otherwise built on stream st, whose completion `ready` marks.
@@ -904,16 +918,17 @@ int ensure_keys(Mesh *m, size_t cap) {
 // host-driven path then sizes them from the counts)
 constexpr int EZ_RETRY = -1;
 
-// Capacities of a device-selection populate.  Candidates (cells at or above
-// the threshold bin, a few times ncell <= ntot) and tracers (ntot and the
-// ties of the PDF walk) are bounded in terms of eval_pdf's ntot, which
-// follows the expected number of tracers and barely the bias parameters, so
-// that one allocation serves a sweep of parameters: the current buffers are
-// kept while they hold 2 ntot + 64 Ki keys (at most N) and ntot + ntot/32 +
-// 1024 tracers, else they grow to 3 ntot + 64 Ki and ntot + ntot/8 + 4096.
-// With alloc false: whether the current ones are kept.
+// Capacities of a device-selection populate.  Candidates (the cells of the
+// threshold's log2 bin and above: 1.3 ntot at the median of a sweep of
+// bias parameters, up to 3.4 ntot, and 2.8 ntot for the 512^3 bench mock)
+// and tracers (ntot and the ties of the PDF walk) are bounded in terms of
+// eval_pdf's ntot, which follows the expected number of tracers and barely
+// the bias parameters, so that one allocation serves a sweep: the current
+// buffers are kept while they hold 4 ntot + 64 Ki keys (at most N) and
+// ntot + ntot/32 + 1024 tracers, else they grow to 6 ntot + 64 Ki and
+// ntot + ntot/8 + 4096.  With alloc false: whether the current ones are kept.
 bool dsel_capacity(Mesh *m, size_t N, size_t ntot, bool alloc, int *e) {
-  size_t kacc = 2 * ntot + 65536, kcap = 3 * ntot + 65536;
+  size_t kacc = 4 * ntot + 65536, kcap = 6 * ntot + 65536;
   if (kacc > N) kacc = N;
   if (kcap > N) kcap = N;
   const size_t cacc = ntot + ntot / 32 + 1024;
@@ -1056,6 +1071,7 @@ int dsel_launch(EZMOCK *ez, Mesh *m, bool vel) {
 // the PDF of the instance.
 int dsel_collect(Mesh *m) {
   const PopCtl &r = m->pio->out;
+  m->last_cnt = r.cnt;
   if (r.flags & POP_RHO_LESS) return EZMOCK_ERR_RHO_LESS;
   if (r.flags & (POP_KEYS_FULL | POP_CAT_FULL)) return EZ_RETRY;
   if ((r.flags & (POP_NO_BIN | POP_WALK_ERR | POP_CAND_ERR)) || m->pio->cflag)
@@ -1125,7 +1141,7 @@ int populate_device(EZMOCK *ez, const real *params, size_t nexp,
     // the bias model runs: at most 3 and 6 per tracer pair, with a margin
     // over the expected count (a larger final count rebuilds them)
     const uint64_t nb = ntot + ntot / 32 + 4096;
-    hipStream_t sp = m->profile ? m->st : m->smt;   // timed runs: one stream
+    hipStream_t sp = mt_ahead_stream(m);
     if ((e = mt_windows(ez, m, 2, 5 * (uint64_t) N, 3 * nb, sp)) ||
         (e = mt_windows(ez, m, 3, 8 * (uint64_t) N, 6 * ((nb + 1) / 2), sp)))
       return e;
@@ -1440,7 +1456,7 @@ int field_launch(EZMOCK *ez, Mesh *m, real *delta, bool fixamp, bool iphase,
     // the bias draws' windows (at most N + 1 draws from 2N) on their own
     // stream while the field is made
     if (RNG(ez)->type == 1 && (e = mt_windows(ez, m, 1, 2 * (uint64_t) N, (uint64_t) N + 1,
-        m->profile ? m->st : m->smt)))
+        mt_ahead_stream(m))))
       return e;
     if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return e;
     Scope s(m, "gen_spectrum");
@@ -1542,16 +1558,20 @@ Mesh::GraphKey graph_key(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
 // Nothing is captured where a replay could not reproduce the plain calls
 // without a host round trip: MT19937 streams, timers on, fields by
 // reference, CIC by tile scan or exact ranges, FFT schedule not yet tuned.
+bool capture_ok(EZMOCK *ez, Mesh *m) {
+  const int Ng = CONF(ez)->Ng;
+  bool tuned = false;
+  (void) fft_schedule(Ng, &tuned);
+  return m->use_graph && !m->profile && m->devsel == 1 && RNG(ez)->type == 0 && !m->psi_ref &&
+      m->mom_valid && m->cic_path == 0 && m->cic_fixed_ok && m->cic_list &&
+      !(m->fft_tune && Ng >= 256 && !tuned) && m->pio && m->ctl_d && m->cic_ovf_h;
+}
+
 void capture_mock(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
   drop_graphs(m);
   Conf *conf = CONF(ez);
   const int Ng = conf->Ng;
-  bool tuned = false;
-  (void) fft_schedule(Ng, &tuned);
-  if (!m->use_graph || m->profile || m->devsel != 1 || RNG(ez)->type != 0 || m->psi_ref ||
-      !m->mom_valid || m->cic_path != 0 || !m->cic_fixed_ok || !m->cic_list ||
-      (m->fft_tune && Ng >= 256 && !tuned) || !m->pio || !m->ctl_d || !m->cic_ovf_h)
-    return;
+  if (!capture_ok(ez, m)) return;
   if (dalloc(&m->seed_d, 1)) return;
   hipGraphExec_t x[2] = {nullptr, nullptr};
   bool ok = true;
@@ -1698,7 +1718,7 @@ int mock_call(EZMOCK *ez, const real *params, const size_t nexp, const bool fixa
 // capture_mock with every other thread's entry into the C ABI held off;
 // skipped when called from inside another entry point of this thread
 void capture_exclusive(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
-  if (t_api_depth != 0) return;
+  if (t_api_depth != 0 || !capture_ok(ez, m)) return;
   std::unique_lock<std::shared_mutex> lk(g_api_mu);
   t_api_depth++;
   capture_mock(ez, m, fixamp, iphase);
@@ -2273,6 +2293,7 @@ int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
 int EZmock_ext_path_counts(EZMOCK *ez, uint64_t *out) {
   if (!ez || !out) return EZMOCK_ERR_ARG_NULL;
   for (int i = 0; i < 5; i++) out[i] = MESH(ez)->npath[i];
+  out[5] = MESH(ez)->last_cnt;
   return EZMOCK_SUCCESS;
 }
 

@@ -18,9 +18,10 @@
 //     that makes room;
 //   * every request is computed by the same calls as a stand-alone mock
 //     (EZmock_set_cosmology, EZmock_setup_linear_pk, EZmock_ext_mock, i.e.
-//     EZmock_create_dens_field + EZmock_ext_populate_device replayed from
-//     the instance's captured HIP graphs, and optionally
-//     EZmock_ext_powspec), so its result does not depend on what else runs.
+//     EZmock_create_dens_field + EZmock_ext_populate_device, with plain
+//     launches unless the pool's option 8 turns on the replay of captured
+//     HIP graphs, and optionally EZmock_ext_powspec), so its result does not
+//     depend on what else runs.
 // Written against the public C API only.
 #include <algorithm>
 #include <chrono>
@@ -69,6 +70,7 @@ struct EZMOCK_POOL {
   std::vector<double> k, P;
   double z = 0, z_pk = 0, Om = 0, Onu = 0, w = -1;
   size_t budget = 0;
+  std::vector<std::pair<int, long>> options;    // EZmock_ext_set_option of every instance
   // scheduling
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -100,6 +102,11 @@ int init_inst(EZMOCK_POOL *p, Inst &in, uint64_t seed) {
   if (!in.ez) return err ? err : EZMOCK_ERR_UNKNOWN;
   // keep the CIC lists within the footprint's 24 bytes per cell
   EZmock_ext_set_option(in.ez, 5, (long) (24 * (size_t) in.Ng * in.Ng * in.Ng));
+  // plain launches: with four instances replaying HIP graphs at once the
+  // pool ran a 64 x 256^3 sweep at 708 mocks/s, with plain launches (and
+  // the selection on the device) at 863
+  EZmock_ext_set_option(in.ez, 8, 0);
+  for (auto &o : p->options) EZmock_ext_set_option(in.ez, o.first, o.second);
   EZmock_set_cosmology(in.ez, 0, 0, true, p->z, p->z_pk, p->Om, p->Onu, p->w, &err);
   EZmock_setup_linear_pk(in.ez, p->k.data(), (int) p->k.size(), p->P.data(),
       nullptr, 0, false, &err);
@@ -265,6 +272,14 @@ EZMOCK_POOL *EZmock_ext_pool_init(const int nslot, const size_t mem_budget,
   p->budget = mem_budget;
   for (int i = 0; i < nslot; i++) p->threads.emplace_back(worker, p);
   return p;
+}
+
+int EZmock_ext_pool_option(EZMOCK_POOL *p, const int option, const long value) {
+  if (!p) return EZMOCK_ERR_ARG_NULL;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->insts.empty() || p->next_id != 1) return EZMOCK_ERR_ARG_PARAMS;
+  p->options.emplace_back(option, value);
+  return EZMOCK_SUCCESS;
 }
 
 long EZmock_ext_pool_submit(EZMOCK_POOL *p, const EZMOCK_REQUEST *req, int *err) {

@@ -31,7 +31,7 @@ def _structs(real):
 
 class MockPool:
     def __init__(self, k, Pk, z_out, Omega_m, z_pk=0.0, Omega_nu=0.0, w=-1.0,
-                 nslot=4, mem_budget=200e9, precision="f32"):
+                 nslot=4, mem_budget=200e9, precision="f32", options=None):
         self._L = _lib.load(precision)
         self._rt = np.float32 if precision == "f32" else np.float64
         self._Req, self._Res = _structs(self._L._real)
@@ -44,6 +44,9 @@ class MockPool:
                                                float(w), C.byref(err))
         if not self._p:
             raise RuntimeError(self._L.EZmock_errmsg(err.value).decode())
+        for o, v in (options or {}).items():        # EZmock.set_option of every instance
+            if self._L.EZmock_ext_pool_option(self._p, int(o), int(v)):
+                raise ValueError(f"pool option {o} = {v} refused")
         self._nbin = {}
 
     def footprint(self, Ngrid, with_powspec=False):
