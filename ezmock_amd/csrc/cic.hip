@@ -352,13 +352,19 @@ __device__ __forceinline__ void cic_deposit(double *acc, const ParticleCell &c,
     return;
   const real wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
   const real qv[3] = {q0, q1, q2};
-  // the eight corners, in an order rotated by lane: lanes whose particles
-  // share a cell (dense regions) then add to different cells in the same
-  // LDS instruction instead of serialising on one address
-  const int rot = (int) (threadIdx.x & 7);
+  // the eight corners in an order permuted by lane: the two low lane bits
+  // flip the corner's x and y sides, so that lanes whose particles share a
+  // cell (dense regions) add to different cells in the same LDS instruction
+  // instead of serialising on one address, while every lane of an
+  // instruction takes the same z side (z-neighbouring cells are
+  // bank-neighbours, and a lane-dependent z side puts pairs of lanes of
+  // consecutive particles on one bank): tiles 3.28 -> 3.19 ms at 512^3
+  // against rotating all three sides by lane, 3.47-3.55 ms with z flipped
+  // by lane or no permutation
+  const int rot = (int) ((threadIdx.x & 3) << 1);
 #pragma unroll
   for (int ci = 0; ci < 8; ci++) {
-    const int cc = (ci + rot) & 7;
+    const int cc = ci ^ rot;
     const int a = cc >> 2, b = (cc >> 1) & 1, d = cc & 1;
     const int la = a ? lx[1] : lx[0], lb = b ? ly[1] : ly[0], ld = d ? lz[1] : lz[0];
     if (la < 0 || lb < 0 || ld < 0) continue;
