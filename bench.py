@@ -42,6 +42,7 @@ def algorithmic_bytes(kernel, Ng, s, ntr):
     table = {
         "gen_spectrum": F,                     # write the spectrum once
         "fft_pass_i": 4 * F,                   # read the spectrum, write 3 components
+        "fft_pass_i_gen": 3 * F,               # generate the spectrum, write 3 components
         "fft_pass_j": 2 * F,
         "fft_rows_c2r": 2 * F,
         "bias_count": F,

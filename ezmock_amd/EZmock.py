@@ -389,12 +389,13 @@ def fft_r2c(x, precision="f32"):
 
 def fft_schedule(Ng, precision="f32"):
     """FFT schedule of grid size Ng in this process (EZmock_ext_fft_schedule):
-    dict(tk_j, tk_i, split, tuned)."""
+    dict(tk_j, tk_i, split, tuned, tk_g)."""
     L = _lib.load(precision)
-    out = np.zeros(4, dtype=np.int32)
+    out = np.zeros(5, dtype=np.int32)
     if L.EZmock_ext_fft_schedule(int(Ng), out.ctypes.data):
         raise RuntimeError(f"no FFT schedule for grid size {Ng}")
-    return dict(tk_j=int(out[0]), tk_i=int(out[1]), split=int(out[2]), tuned=bool(out[3]))
+    return dict(tk_j=int(out[0]), tk_i=int(out[1]), split=int(out[2]), tuned=bool(out[3]),
+                tk_g=int(out[4]))
 
 
 def fft_set_schedule(Ng, tk_j, tk_i, split, precision="f32"):

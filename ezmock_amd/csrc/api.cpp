@@ -153,6 +153,7 @@ struct Mesh {
     uint64_t cnt[EZ_MAX_NUM_TRACER_PER_CELL + 1];
     unsigned long long cflag;
   } *pio = nullptr;
+  bool gen_fused = true;                // spectrum generated inside the first FFT pass
   int devsel = 1;                       // 2: capacities of half the expected counts
   // path counters: device-selection populates, host-driven ones, capacity
   // retries, graph replays, graph captures
@@ -167,14 +168,14 @@ struct Mesh {
     bool fixamp = false, iphase = false, keep_spec = false, fft_streams = false;
     rkey *keys = nullptr; size_t keys_cap = 0; real *cat0 = nullptr; size_t cat_cap = 0;
     uint32_t *cic_list = nullptr; size_t cic_list_cap = 0; uint32_t fcap = 0;
-    int cic_path = 0; bool cic_fixed_ok = false;
+    int cic_path = 0; bool cic_fixed_ok = false, gen_fused = false;
     int sched[3] = {0, 0, 0};
     bool operator==(const GraphKey &o) const {
       return pk_version == o.pk_version && vfac == o.vfac && fixamp == o.fixamp &&
           iphase == o.iphase && keep_spec == o.keep_spec && fft_streams == o.fft_streams &&
           keys == o.keys && keys_cap == o.keys_cap && cat0 == o.cat0 && cat_cap == o.cat_cap &&
           cic_list == o.cic_list && cic_list_cap == o.cic_list_cap && fcap == o.fcap &&
-          cic_path == o.cic_path && cic_fixed_ok == o.cic_fixed_ok &&
+          cic_path == o.cic_path && cic_fixed_ok == o.cic_fixed_ok && gen_fused == o.gen_fused &&
           sched[0] == o.sched[0] && sched[1] == o.sched[1] && sched[2] == o.sched[2];
     }
   } gkey;
@@ -1425,6 +1426,7 @@ int field_launch(EZMOCK *ez, Mesh *m, real *delta, bool fixamp, bool iphase,
   const int Ngh = Ng / 2;
   const size_t plane = (size_t) Ng * Ng;
   int e;
+  bool gen_fused = false, mrg_ic = false;
   GenParams gp;
   gp.Ng = Ng;
   gp.Nh = plane * (Ngh + 1);
@@ -1458,10 +1460,24 @@ int field_launch(EZMOCK *ez, Mesh *m, real *delta, bool fixamp, bool iphase,
     if (RNG(ez)->type == 1 && (e = mt_windows(ez, m, 1, 2 * (uint64_t) N, (uint64_t) N + 1,
         mt_ahead_stream(m))))
       return e;
-    if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return e;
-    Scope s(m, "gen_spectrum");
-    KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
-        m->st, seedp));
+    // MRG32k3a with a tuned, fused first pass: the spectrum is generated by
+    // the first FFT pass itself, tile by tile, and only its kz = 0 and
+    // Nyquist planes here (they enter the pass through the packed slots)
+    bool tuned = false;
+    const FftSchedule fs = fft_schedule(Ng, &tuned);
+    mrg_ic = !mtw;
+    gen_fused = !mtw && !spec && m->gen_fused && fs.split == 0 &&
+        (tuned || !m->fft_tune || Ng < 256);
+    if (gen_fused) {
+      Scope s(m, "gen_planes");
+      KCHK(launch_gen_planes(rhok, m->ny + plane, gp, m->ptab, m->jtab, m->st, seedp));
+    }
+    else {
+      if (!mtw && (e = dalloc(&m->gen_base, gen_base_count(gp.Nh)))) return e;
+      Scope s(m, "gen_spectrum");
+      KCHK(launch_gen_rhok(rhok, m->ny + plane, spec, gp, m->ptab, m->jtab, mtw, m->gen_base,
+          m->st, seedp));
+    }
   }
   // the generated Nyquist plane lives in ny[plane, 2 plane)
   cplx *ny_save = m->ny;
@@ -1480,11 +1496,20 @@ int field_launch(EZMOCK *ez, Mesh *m, real *delta, bool fixamp, bool iphase,
   (void) fft_schedule(Ng, &tuned);
   if (m->fft_tune && Ng >= 256 && !tuned &&
       fft_autotune(Ng, (cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2], rhok,
-          m->ny, m->slot0, m->twN, m->st, nullptr, nullptr) != hipSuccess) {
+          m->ny, m->slot0, m->twN, m->st, nullptr, nullptr, mrg_ic ? &gp : nullptr, m->ptab,
+          m->jtab) != hipSuccess) {
     m->ny = ny_save;
     return EZMOCK_ERR_UNKNOWN;
   }
-  {
+  if (gen_fused) {
+    Scope s(m, "fft_pass_i_gen");
+    if (fft_first_pass3_gen((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
+        m->slot0, m->twN, Ng, gp, m->ptab, m->jtab, seedp, m->st) != hipSuccess) {
+      m->ny = ny_save;
+      return EZMOCK_ERR_UNKNOWN;
+    }
+  }
+  else {
     Scope s(m, "fft_pass_i");
     if (fft_first_pass3((cplx *) m->psi[0], (cplx *) m->psi[1], (cplx *) m->psi[2],
         rhok, m->ny, m->slot0, m->twN, Ng, m->st) != hipSuccess) {
@@ -1544,6 +1569,7 @@ Mesh::GraphKey graph_key(EZMOCK *ez, Mesh *m, bool fixamp, bool iphase) {
   k.fcap = m->cic_fcap;
   k.cic_path = m->cic_path;
   k.cic_fixed_ok = m->cic_fixed_ok;
+  k.gen_fused = m->gen_fused;
   bool tuned = false;
   const FftSchedule fs = fft_schedule(CONF(ez)->Ng, &tuned);
   k.sched[0] = fs.tk_j;
@@ -2319,6 +2345,7 @@ int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value) {
   else if (option == 4) m->fft_streams = value != 0;
   else if (option == 6) m->fft_tune = value != 0;
   else if (option == 7 && value >= 0 && value <= 2) m->devsel = (int) value;
+  else if (option == 9) m->gen_fused = value != 0;
   else if (option == 8) {
     m->use_graph = value != 0;
     if (!m->use_graph) drop_graphs(m);
@@ -2512,6 +2539,7 @@ int EZmock_ext_fft_schedule(const int Ng, int *out) {
   bool tuned = false;
   const FftSchedule sc = fft_schedule(Ng, &tuned);
   out[0] = sc.tk_j; out[1] = sc.tk_i; out[2] = sc.split; out[3] = tuned ? 1 : 0;
+  out[4] = sc.tk_g;
   return EZMOCK_SUCCESS;
 }
 
@@ -2521,7 +2549,7 @@ int EZmock_ext_fft_set_schedule(const int Ng, const int tk_j, const int tk_i,
   if (!fft_supported(Ng)) return EZMOCK_ERR_ARG_NGRID;
   auto ok = [](int t) { return t == 0 || t == 4 || t == 8 || t == 16; };
   if (!ok(tk_j) || !ok(tk_i) || (split != 0 && split != 1)) return EZMOCK_ERR_ARG_PARAMS;
-  fft_set_schedule(Ng, FftSchedule{tk_j, tk_i, split});
+  fft_set_schedule(Ng, FftSchedule{tk_j, tk_i, split, 0});
   return EZMOCK_SUCCESS;
 }
 

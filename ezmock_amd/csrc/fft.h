@@ -29,7 +29,10 @@ hipError_t fft_slot0_pack(const cplx *rhok, const cplx *rhok_ny, cplx *slot0,
 // the first pass (tk_i; 0 = defaults), and the first pass fused (split = 0:
 // the spectrum read once, three transforms per workgroup) or split into
 // three single-component strided passes (split = 1).
-struct FftSchedule { int tk_j, tk_i, split; };
+// tk_g: tile width of the first pass that generates its spectrum
+// (fft_first_pass3_gen; 0 = that of the first pass).
+struct FftSchedule { int tk_j, tk_i, split, tk_g; };
+struct GenParams;
 FftSchedule fft_schedule(int Ng, bool *tuned);
 void fft_set_schedule(int Ng, const FftSchedule &s);
 
@@ -40,9 +43,21 @@ hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
     const cplx *rhok_ny, const cplx *slot0, const cplx *tw, int Ng, hipStream_t st,
     const FftSchedule *sched = nullptr);
 
+// The same first pass generating the spectrum (MRG32k3a initial
+// conditions, gp) tile by tile instead of reading rhok; the kz = 0 and
+// Nyquist planes come through slot0 (from launch_gen_planes and
+// fft_slot0_pack).  hipErrorNotSupported when the grid's schedule splits
+// the first pass.  seedp (optional): the seed in device memory.
+hipError_t fft_first_pass3_gen(cplx *out0, cplx *out1, cplx *out2, const cplx *slot0,
+    const cplx *tw, int Ng, const GenParams &gp, const double *ptab, const uint32_t *jtab,
+    const struct MrgState *seedp, hipStream_t st, int tk = -1);
+
 // Times every candidate schedule of grid Ng on the given buffers (out0..2
 // overwritten, the spectrum read) and records the fastest for the grid;
-// ms_best[0], [1]: best times of the j pass and of the first pass.
+// ms_best[0], [1]: best times of the j pass and of the first pass.  With gp
+// (MRG32k3a initial conditions), also the tile widths of the first pass that
+// generates its spectrum (tk_g).
 hipError_t fft_autotune(int Ng, cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
     const cplx *rhok_ny, const cplx *slot0, const cplx *tw, hipStream_t st,
-    FftSchedule *best, double *ms_best);
+    FftSchedule *best, double *ms_best,
+    const GenParams *gp = nullptr, const double *ptab = nullptr, const uint32_t *jtab = nullptr);

@@ -24,6 +24,8 @@
 
 #include "common.h"
 #include "fft.h"
+#include "rng.h"
+#include "gen.h"
 
 // ------------------------------------------------------------ radix plan
 __host__ __device__ constexpr int pow2_exp(int L) {
@@ -384,6 +386,93 @@ k_fft_first3(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
   }
 }
 
+// The same pass generating its spectrum tile itself (MRG32k3a, initial
+// conditions): each thread draws whole rows of the tile, row i holding the
+// cells (i, j, kk0 .. kk0 + TK - 1), whose draws are consecutive (one
+// jump-ahead per row), and stages them in LDS (slot c of row n at
+// c ^ (n mod TK), so that the lanes writing one column of rows hit distinct
+// banks); the kz = 0 column comes from slot0 as in k_fft_first3, whose
+// planes k_gen_planes generated.  Saves writing and reading the spectrum.
+template <class C, int L, int TK, int NT>
+__global__ void __launch_bounds__(NT)
+k_fft_first3_gen(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out2,
+    const C *__restrict__ slot0, const C *__restrict__ tw, int Ng, int Ngh, GenParams gp,
+    const double *__restrict__ ptab, const uint32_t *__restrict__ jtab,
+    const MrgState *__restrict__ seedp) {
+  __shared__ C lds[L * TK];
+  constexpr int E = L * TK;
+  constexpr int IT = (E + NT - 1) / NT;
+  const unsigned nx = (unsigned) (Ngh / TK);
+  const unsigned t = xcd_item(blockIdx.x, gridDim.x);
+  const int kk0 = (int) (t % nx) * TK;
+  const int o = (int) (t / nx);             // j
+  const size_t stride = (size_t) Ng * Ngh;
+  const size_t base = (size_t) o * Ngh + kk0;
+  const int ko = (o <= Ngh) ? o : o - Ng;
+  {
+    const MrgState seed = seedp ? *seedp : gp.seed_state;
+    const uint64_t mult = gp.fixamp ? 1 : 2;
+    const size_t Ngk = (size_t) Ngh + 1;
+    const int c0 = (kk0 == 0) ? 1 : 0;      // kz = 0: from slot0
+#pragma unroll 1
+    for (int n = threadIdx.x; n < L; n += NT) {
+      C *row = lds + n * TK;
+      const int sw = n & (TK - 1);
+      if (c0) row[sw] = C{0, 0};
+      MrgState g = seed;
+      mrg_jump(g, (((size_t) n * Ng + o) * Ngk + kk0 + c0 - 1) * mult, jtab);
+#pragma unroll 1
+      for (int c = c0; c < TK; c++) {
+        const double amp = gp.fixamp ? 1.0 : mrg_next(g);
+        const double ph = mrg_next(g);
+        const GenCell gc = gen_cell(n, o, kk0 + c, amp, ph, gp, ptab);
+        row[c ^ sw] = C{gc.re, gc.im};
+      }
+    }
+  }
+  __syncthreads();
+  C h[IT];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * NT;
+    if (IT * NT == E || e < E) {
+      const int n = e / TK, c = e % TK;
+      h[it] = lds[n * TK + (c ^ (n & (TK - 1)))];
+    }
+  }
+  __syncthreads();
+  C *outs[3] = {out0, out1, out2};
+#pragma unroll 1
+  for (int comp = 0; comp < 3; comp++) {
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      if (IT * NT == E || e < E) {
+        const int n = e / TK, c = e % TK;
+        const int kn = (n <= Ngh) ? n : n - Ng;
+        lds[n * TK + c] = spec_mul(h[it], comp, kn, ko, kk0 + c);
+      }
+    }
+    if (kk0 == 0) {                         // uniform over the block
+      const C *col = slot0 + ((size_t) comp * Ng + o) * Ng;
+      __syncthreads();
+      for (int n = threadIdx.x; n < L; n += NT) lds[n * TK] = col[n];
+    }
+    __syncthreads();
+    lds_fft<C, L, TK, NT, +1, TK, 1, true>(lds, tw);
+    C *out = outs[comp];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      if (IT * NT == E || e < E) {
+        const int n = e / TK, c = e % TK;
+        out[base + (size_t) n * stride + c] = lds[n * TK + c];
+      }
+    }
+    __syncthreads();                        // before the next component's fill
+  }
+}
+
 // ------------------------------------------------------------ row passes
 // c2r along kk: in place, rows of Lh complex -> 2 Lh reals.
 // twN[m] = exp(+2 pi i m / (2 Lh)), twh[m] = exp(+2 pi i m / Lh).
@@ -594,6 +683,18 @@ static hipError_t launch_first3_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *rho
   return hipGetLastError();
 }
 
+template <int L, int T>
+static hipError_t launch_first3_gen_tk(cplx *o0, cplx *o1, cplx *o2, const cplx *slot0,
+    const cplx *tw, int Ng, int Ngh, const GenParams &gp, const double *ptab,
+    const uint32_t *jtab, const MrgState *seedp, hipStream_t st) {
+  constexpr int E = L * T;
+  constexpr int NT = (E / 16 > 512) ? 512 : (E / 16 < 64 ? 64 : E / 16);
+  hipLaunchKernelGGL((k_fft_first3_gen<cplx, L, T, NT>),
+      dim3((unsigned) ((Ngh / T) * (size_t) Ng)), dim3(NT), 0, st, o0, o1, o2, slot0, tw,
+      Ng, Ngh, gp, ptab, jtab, seedp);
+  return hipGetLastError();
+}
+
 // The three-component pass keeps its tile in registers and runs three
 // transforms per workgroup: the narrowest power-of-two tile whose LDS
 // reaches 64 KB (at most 16 columns; measured at 512, 1024 and 1536).
@@ -629,6 +730,33 @@ static hipError_t launch_first3(cplx *o0, cplx *o1, cplx *o2, const cplx *rhok,
         return launch_first3_tk<L, 16>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
   }
   return launch_first3_default<L>(o0, o1, o2, rhok, slot0, tw, Ng, Ngh, st);
+}
+
+template <int L>
+static hipError_t launch_first3_gen(cplx *o0, cplx *o1, cplx *o2, const cplx *slot0,
+    const cplx *tw, int Ng, int Ngh, int tk, const GenParams &gp, const double *ptab,
+    const uint32_t *jtab, const MrgState *seedp, hipStream_t st) {
+  if constexpr (tunable<L>()) {
+    if constexpr (tk_fits<L, 4>())
+      if (tk == 4 && Ngh % 4 == 0)
+        return launch_first3_gen_tk<L, 4>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+    if constexpr (tk_fits<L, 8>())
+      if (tk == 8 && Ngh % 8 == 0)
+        return launch_first3_gen_tk<L, 8>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+    if constexpr (tk_fits<L, 16>())
+      if (tk == 16 && Ngh % 16 == 0)
+        return launch_first3_gen_tk<L, 16>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+  }
+  constexpr int T = First3Cfg<L>::TKMAX;
+  if (Ngh % T == 0)
+    return launch_first3_gen_tk<L, T>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+  if constexpr (T >= 8) if (Ngh % (T / 2) == 0)
+    return launch_first3_gen_tk<L, T / 2>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+  if constexpr (T >= 4) if (Ngh % 4 == 0)
+    return launch_first3_gen_tk<L, 4>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+  if (Ngh % 2 == 0)
+    return launch_first3_gen_tk<L, 2>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
+  return launch_first3_gen_tk<L, 1>(o0, o1, o2, slot0, tw, Ng, Ngh, gp, ptab, jtab, seedp, st);
 }
 
 template <int Lh>
@@ -738,6 +866,23 @@ hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
   }
 }
 
+hipError_t fft_first_pass3_gen(cplx *out0, cplx *out1, cplx *out2, const cplx *slot0,
+    const cplx *tw, int Ng, const GenParams &gp, const double *ptab, const uint32_t *jtab,
+    const MrgState *seedp, hipStream_t st, int tk) {
+  const int Ngh = Ng / 2;
+  if (tk < 0) {
+    const FftSchedule sc = fft_schedule(Ng, nullptr);
+    if (sc.split) return hipErrorNotSupported;
+    tk = sc.tk_g ? sc.tk_g : sc.tk_i;
+  }
+  switch (Ng) {
+#define X(n) case n: return launch_first3_gen<n>(out0, out1, out2, slot0, tw, Ng, Ngh, tk, gp, ptab, jtab, seedp, st);
+    EZ_FFT_SIZES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // Times every candidate schedule of grid Ng on the caller's buffers (the
 // spectrum rhok / rhok_ny / slot0 is read; out0..out2 are overwritten) and
 // records the fastest: tile widths 4, 8 and 16 (where they fit), the first
@@ -745,7 +890,8 @@ hipError_t fft_first_pass3(cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
 // two timed runs.
 hipError_t fft_autotune(int Ng, cplx *out0, cplx *out1, cplx *out2, const cplx *rhok,
     const cplx *rhok_ny, const cplx *slot0, const cplx *tw, hipStream_t st,
-    FftSchedule *best, double *ms_best) {
+    FftSchedule *best, double *ms_best, const GenParams *gp, const double *ptab,
+    const uint32_t *jtab) {
   hipEvent_t a, b;
   hipError_t e;
   if ((e = hipEventCreate(&a)) != hipSuccess) return e;
@@ -788,6 +934,18 @@ hipError_t fft_autotune(int Ng, cplx *out0, cplx *out1, cplx *out2, const cplx *
       if (e != hipSuccess) break;
       if (ms < best_i) { best_i = ms; bs.tk_i = t; bs.split = split; }
     }
+  }
+  // the first pass generating its spectrum (used with the fused first pass)
+  float best_g = 1e30f;
+  for (int t : tks) {
+    if (!gp || e != hipSuccess) break;
+    if (t && Ngh % t) continue;
+    if (t * Ng * (int) sizeof(cplx) > 160 * 1024) continue;
+    float ms;
+    e = time_it([&] { return fft_first_pass3_gen(out0, out1, out2, slot0, tw, Ng, *gp, ptab,
+        jtab, nullptr, st, t); }, ms);
+    if (e != hipSuccess) break;
+    if (ms < best_g) { best_g = ms; bs.tk_g = t; }
   }
   (void) hipEventDestroy(a);
   (void) hipEventDestroy(b);

@@ -7,6 +7,7 @@
 #include "common.h"
 #include "rng.h"
 #include "kernels.h"
+#include "gen.h"
 
 // ------------------------------------------------------------ P(k) table
 // linear_pk.c:36-51 (bin search) and math/cspline.c:91-105 (evaluation).
@@ -64,68 +65,6 @@ __global__ void k_pk_table(double *__restrict__ tab, int nsq,
 #define GEN_SKIP 0xffffu
 
 #define GEN_CHUNK (sizeof(real) == 4 ? 16 : 8)     // cells per thread
-
-// One cell (i, j, k) of the generated spectrum from its two draws (amp in
-// (0, 1], phase draw ph01 in [0, 1)), perturb.c:78-359: whether this cell
-// is written (cells of the kz = 0 and Nyquist planes outside the lower half
-// are filled by their mirror), whether it also writes its mirror (ni, nj),
-// and its value.
-struct GenCell {
-  real re, im;
-  bool write, mirror;
-  int ni, nj;
-};
-
-__device__ __forceinline__ GenCell gen_cell(int i, int j, int k, double amp, double ph01,
-    const GenParams &gp, const double *__restrict__ ptab) {
-  const int Ng = gp.Ng, Ngh = Ng >> 1;
-  GenCell c{0, 0, true, false, 0, 0};
-  double ph = ph01 * 2 * EZ_PI;
-  const bool plane_k = (k == 0) || (k == Ngh);
-  if (plane_k) {
-    c.ni = (i == 0) ? 0 : Ng - i;
-    c.nj = (j == 0) ? 0 : Ng - j;
-    const bool lower = (i == c.ni && j <= Ngh) || (i != c.ni && i <= Ngh);
-    if (!lower) c.write = false;      // filled by its mirror
-    c.mirror = lower;
-  }
-  if (!c.write) return c;
-  const int ki = (i <= Ngh) ? i : i - Ng;
-  const int kj = (j <= Ngh) ? j : j - Ng;
-  const int ksqi = ki * ki + kj * kj + k * k;
-  const double ksq = (double) ksqi;
-  double P = ptab[ksqi];
-  double a = gp.fixamp ? 1.0 : -log(amp);
-  if (gp.logpk) P = gp.fixamp ? exp(P * 0.5) : exp(P * 0.5) * sqrt(a);
-  else P = gp.fixamp ? sqrt(P) : sqrt(P * a);
-  P *= gp.Pfac;
-  if (gp.iphase) ph += EZ_PI;
-  P /= (ksq * gp.kfac);
-  double sn, cs;
-  sincos(ph, &sn, &cs);                 // one argument reduction for both
-  c.re = (real) (-P * sn);
-  c.im = (real) (P * cs);
-  // perturb.c:341-359: self-conjugate modes keep only the imaginary part
-  // (the reference writes (+P sin, P cos) twice, then zeroes re).
-  if (c.mirror && i == c.ni && j == c.nj) c.re = 0;
-  return c;
-}
-
-// The mirror of a written kz = 0 / Nyquist cell (not for self-conjugate
-// modes): (-re, im) at (ni, nj, k).
-__device__ __forceinline__ void gen_mirror(const GenCell &c, int i, int j, int k,
-    const GenParams &gp, cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny,
-    real *__restrict__ ref_copy) {
-  if (!c.mirror || (i == c.ni && j == c.nj)) return;
-  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
-  const cplx w = {-c.re, c.im};
-  if (k < Ngh) rhok[((size_t) c.ni * Ng + c.nj) * Ngh + k] = w;
-  else rhok_ny[(size_t) c.ni * Ng + c.nj] = w;
-  if (ref_copy) {
-    const size_t nidx = ((size_t) c.ni * Ng + c.nj) * Ngk + k;
-    ref_copy[2 * nidx] = -c.re; ref_copy[2 * nidx + 1] = c.im;
-  }
-}
 
 // MRG32k3a state at the first draw of each k_gen_rhok workgroup, one thread
 // per workgroup, so that no workgroup waits on a serial jump of its own.
@@ -258,6 +197,50 @@ hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     hipLaunchKernelGGL((k_gen_rhok<CH>), dim3(nblk), dim3(GEN_TPB), 0,
         st, rhok, rhok_ny, ref_copy, gp, ptab, jtab, (const MrgState *) bases);
   }
+  return hipGetLastError();
+}
+
+// The kz = 0 and Nyquist planes of the spectrum only (rhok[i][j][0] and
+// rhok_ny), one thread per (i, j), each cell from its own draws by a
+// jump-ahead: the rest of the spectrum is generated by the first FFT pass
+// itself (fft_first_pass3_gen), which reads these two planes through the
+// packed kz = 0 slots.  seedp (optional): the seed in device memory.
+__global__ void __launch_bounds__(256)
+k_gen_planes(cplx *__restrict__ rhok, cplx *__restrict__ rhok_ny, GenParams gp,
+    const double *__restrict__ ptab, const uint32_t *__restrict__ jtab,
+    const MrgState *__restrict__ seedp) {
+  const int Ng = gp.Ng, Ngh = Ng >> 1, Ngk = Ngh + 1;
+  const size_t row = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (size_t) Ng * Ng) return;
+  const int i = (int) (row / (size_t) Ng), j = (int) (row % (size_t) Ng);
+  const uint64_t mult = gp.fixamp ? 1 : 2;
+  const MrgState seed = seedp ? *seedp : gp.seed_state;
+#pragma unroll 1
+  for (int h = 0; h < 2; h++) {
+    const int k = h ? Ngh : 0;
+    const size_t idx = row * (size_t) Ngk + k;
+    GenCell gc{0, 0, true, false, 0, 0};
+    if (idx != 0) {
+      MrgState g = seed;
+      mrg_jump(g, (idx - 1) * mult, jtab);
+      const double amp = gp.fixamp ? 1.0 : mrg_next(g);
+      const double ph = mrg_next(g);
+      gc = gen_cell(i, j, k, amp, ph, gp, ptab);
+    }
+    if (!gc.write) continue;
+    const cplx v = {gc.re, gc.im};
+    if (h) rhok_ny[row] = v;
+    else rhok[row * (size_t) Ngh] = v;
+    gen_mirror(gc, i, j, k, gp, rhok, rhok_ny, nullptr);
+  }
+}
+
+hipError_t launch_gen_planes(cplx *rhok, cplx *rhok_ny, const GenParams &gp,
+    const double *ptab, const uint32_t *jtab, hipStream_t st, const MrgState *seedp) {
+  if (gp.Ng & 1) return hipErrorInvalidValue;
+  const size_t n = (size_t) gp.Ng * gp.Ng;
+  hipLaunchKernelGGL(k_gen_planes, dim3(ez_blocks(n, 256)), dim3(256), 0, st,
+      rhok, rhok_ny, gp, ptab, jtab, seedp);
   return hipGetLastError();
 }
 

@@ -24,6 +24,9 @@ size_t gen_base_count(size_t Nh);
 hipError_t launch_gen_rhok(cplx *rhok, cplx *rhok_ny, real *ref_copy,
     const GenParams &gp, const double *ptab, const uint32_t *jtab,
     const uint32_t *mtw, MrgState *bases, hipStream_t st, const MrgState *seedp = nullptr);
+// The kz = 0 and Nyquist planes only (for fft_first_pass3_gen), MRG32k3a.
+hipError_t launch_gen_planes(cplx *rhok, cplx *rhok_ny, const GenParams &gp,
+    const double *ptab, const uint32_t *jtab, hipStream_t st, const MrgState *seedp = nullptr);
 hipError_t launch_gen_rhok_wn(cplx *rhok, cplx *rhok_ny, const cplx *dk,
     const cplx *dk_ny, const GenParams &gp, const double *ptab,
     real *ref_copy, hipStream_t st);

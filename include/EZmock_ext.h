@@ -115,7 +115,13 @@ int EZmock_ext_pdf_info(EZMOCK *ez, size_t *out, const size_t n);
                 outgrows them is redone by the host-driven path (0), which
                 sizes them from the counts.  Both give the same catalogue;
             8 = HIP-graph replay in EZmock_ext_mock (1, default; 0 drops
-                the instance's graphs). */
+                the instance's graphs);
+            9 = spectrum generated inside the first FFT pass (1, default,
+                with MRG32k3a initial conditions once the grid's FFT
+                schedule is tuned and its first pass is fused; 0 = the
+                spectrum written by its own kernel and read back).  Both
+                give the same fields; the spectrum copy of option 1 needs
+                the separate kernel. */
 int EZmock_ext_set_option(EZMOCK *ez, const int option, const long value);
 
 /* Per-kernel device time totals (ms) and launch counts since the last reset
@@ -241,10 +247,13 @@ int EZmock_ext_pdf_walk(const real *rhot, const size_t num,
    of the strided (j) passes, out[1] = tile width of the first (i) pass
    (0 = the built-in default, else 4, 8 or 16), out[2] = 1 if the first pass
    runs as three single-component passes instead of the fused
-   three-component pass, out[3] = 1 if chosen by the autotuner (or set).
-   All schedules give bit-identical fields. */
-int EZmock_ext_fft_schedule(const int Ngrid, int *out);
-/* Sets the schedule of grid size Ng (tile widths 0, 4, 8, 16; split 0/1). */
+   three-component pass, out[3] = 1 if chosen by the autotuner (or set),
+   out[4] = tile width of the first pass that generates its own spectrum
+   (option 9; 0 = that of out[1]).  All schedules give bit-identical
+   fields. */
+int EZmock_ext_fft_schedule(const int Ngrid, int *out5);
+/* Sets the schedule of grid size Ng (tile widths 0, 4, 8, 16; split 0/1;
+   the generating first pass takes tk_i). */
 int EZmock_ext_fft_set_schedule(const int Ngrid, const int tk_j, const int tk_i,
     const int split);
 

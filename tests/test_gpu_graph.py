@@ -171,3 +171,35 @@ def test_mock_graph_populate_after_replay(gpu_available, pk_table):
     assert a == p.populate_tracer_device(*PARAMS[2], 5000)
     same_catalogue(g.device_catalog(), p.device_catalog(), "after disp")
     assert g.path_counts()["replay"] >= 2
+
+
+@pytest.mark.parametrize("Ng,L,prec,fixamp,iphase", [
+    (32, 320., "f64", False, False), (48, 400., "f64", True, True),
+    (64, 640., "f32", False, False), (64, 640., "f32", True, False),
+    (96, 600., "f32", False, True), (256, 1000., "f32", False, False)])
+def test_spectrum_generated_in_first_fft_pass(gpu_available, pk_table, Ng, L, prec, fixamp,
+                                              iphase):
+    """The spectrum generated inside the first FFT pass (option 9, default;
+    the kz = 0 and Nyquist planes by their own kernel) against the
+    spectrum kernel whose output the pass reads (option 9 = 0): identical
+    displacement fields, for two seeds (at 256^3 the first field of the
+    grid size times the FFT schedules on the separate kernel's spectrum)."""
+    f = made(pk_table, Ng, L, 5, prec)
+    s = made(pk_table, Ng, L, 5, prec, {9: 0})
+    for seed in (5, 6):
+        for e in (f, s):
+            e.set_seed(seed)
+            e.create_dens_field_from_ic(fixamp=fixamp, iphase=iphase)
+        for c in ("psi_x", "psi_y", "psi_z"):
+            assert np.array_equal(f.field(c), s.field(c)), (seed, c)
+        rf, rs = f.field("rho"), s.field("rho")
+        assert np.abs(rf - rs).max() <= (1e-12 * np.abs(rs).max() if prec == "f64" else 0)
+    # and the whole mock, replayed from graphs, with the separate kernel
+    n = Ng ** 3 // 50
+    for e in (f, s):
+        e.set_seed(9)
+        e.mock(0.5, 10., 0.3, 150., n)
+        e.set_seed(10)
+        e.mock(0.5, 10., 0.3, 150., n)
+    assert f.path_counts()["replay"] >= 1
+    assert np.array_equal(f.field("psi_y"), s.field("psi_y"))
