@@ -145,6 +145,7 @@ struct Mesh {
   // staging (parameters in, results out), spectrum seed of a replayed mock
   PopCtl *ctl_d = nullptr;
   MrgState *seed_d = nullptr;
+  unsigned long long *sub16 = nullptr;  // candidate counts by sub-bin
   struct PopIO {
     PopCtl in;
     MrgState seed;
@@ -267,7 +268,7 @@ void drop_graphs(Mesh *m) {
 void free_mesh_device(Mesh *m) {
   if (m->smt) (void) hipStreamSynchronize(m->smt);   // window builds ahead of use
   drop_graphs(m);
-  dfree(&m->ctl_d); dfree(&m->seed_d);
+  dfree(&m->ctl_d); dfree(&m->seed_d); dfree(&m->sub16);
   if (m->pio) { (void) hipHostFree(m->pio); m->pio = nullptr; }
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
   dfree(&m->vel4); dfree(&m->gen_base);
@@ -952,6 +953,11 @@ bool dsel_capacity(Mesh *m, size_t N, size_t ntot, bool alloc, int *e) {
   return true;
 }
 
+// Refined candidates the sort covers: a third of the key capacity (2 ntot
+// + 21 Ki keys at the capacity's growth; 0.95 ntot measured for the 512^3
+// bench mock); more are redone by the host-driven path.
+size_t dsel_kfcap(const Mesh *m) { return m->keys_cap / 3 ? m->keys_cap / 3 : 1; }
+
 // The control block of a device-selection populate into the pinned
 // staging, and its upload with the PDF bin sizes (stream order).  Every
 // call of the engine ends with a synchronisation, so the staging is free
@@ -959,7 +965,7 @@ bool dsel_capacity(Mesh *m, size_t N, size_t ntot, bool alloc, int *e) {
 int dsel_upload(Mesh *m, const BiasParams &bp, const std::vector<size_t> &num, size_t ncell,
     real sigma_v) {
   int e;
-  if ((e = dalloc(&m->ctl_d, 1))) return e;
+  if ((e = dalloc(&m->ctl_d, 1)) || (e = dalloc(&m->sub16, 16))) return e;
   if (!m->pio && hipHostMalloc((void **) &m->pio, sizeof(Mesh::PopIO)) != hipSuccess) {
     m->pio = nullptr;
     return EZMOCK_ERR_MEMORY;
@@ -976,6 +982,7 @@ int dsel_upload(Mesh *m, const BiasParams &bp, const std::vector<size_t> &num, s
   c.sigma_v = (double) sigma_v;
   c.ncell = ncell;
   c.kcap = m->keys_cap;
+  c.kfcap = dsel_kfcap(m);
   c.ccap = m->cat_cap;
   c.nmax = (int) num.size();
   for (size_t p = 0; p < num.size(); p++) m->pio->num[p] = num[p];
@@ -1028,14 +1035,22 @@ int dsel_launch(EZMOCK *ez, Mesh *m, bool vel) {
     KCHK(launch_compact_at(m->rhot, 0, N, m->blockoff, m->blockcnt, m->keys, m->keys_cap,
         m->counter, m->st, ctl));
   }
+  // the candidates at or above a threshold refined inside the threshold
+  // bin, into keys_tmp (about a third of them at 512^3), then sorted there
+  KCHK(launch_zero(m->sub16, 16 * sizeof(unsigned long long), m->st));
+  {
+    Scope s(m, "refine");
+    KCHK(launch_refine(m->keys, m->keys_cap, ctl, m->sub16, m->keys_tmp, m->counter + 1,
+        m->st));
+  }
   {
     Scope s(m, "sort");
-    KCHK(launch_sort(m->keys, m->keys_tmp, m->keys_cap, m->sort_gh, m->sort_go, m->sort_scan,
-        m->st, &ctl->nsort));
+    KCHK(launch_sort(m->keys_tmp, m->keys, dsel_kfcap(m), m->sort_gh, m->sort_go,
+        m->sort_scan, m->st, &ctl->nfine));
   }
   {
     Scope s(m, "walk");
-    KCHK(launch_walk(m->keys, m->keys_cap, m->d_num, 0, m->d_vb, m->d_cnt, m->d_res, m->st,
+    KCHK(launch_walk(m->keys_tmp, m->keys_cap, m->d_num, 0, m->d_vb, m->d_cnt, m->d_res, m->st,
         ctl, m->scan_tmp + nch));
   }
   {
@@ -2304,9 +2319,7 @@ int EZmock_ext_copy_field(EZMOCK *ez, const int which, void *dst, int *err) {
     case 4: src = m->rhot; bytes = N * sizeof(real); break;
     case 5: src = m->nt; bytes = N * sizeof(int16_t); break;
     case 6: src = m->spec; bytes = 2 * (size_t) Ng * Ng * (Ng / 2 + 1) * sizeof(real); break;
-    case 7:
-      memcpy(dst, m->hist_h, sizeof(m->hist_h));
-      return EZMOCK_SUCCESS;
+    case 7: src = m->hist; bytes = EZ_NUM_LOGB_BIN * sizeof(unsigned long long); break;
     default: return (*err = EZMOCK_ERR_ARG_NULL);
   }
   if (!src) return (*err = EZMOCK_ERR_ARG_MESH);

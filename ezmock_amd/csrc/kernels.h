@@ -118,12 +118,17 @@ struct PopCtl {
   double sigma_v;
   uint64_t ncell;               // cells eval_pdf asks for
   uint64_t kcap, ccap;          // capacities: candidate keys, catalogue
+  uint64_t kfcap;               // refined candidates the sort's grid covers
   int nmax;
   // written by the kernels
   int imax;                     // bin where the cumulative histogram reaches ncell
   real rho_thres;
   uint64_t nvalid, cnt, nsort, ncell_out, ntot;
   uint32_t flags;               // POP_* bits
+  // candidates kept for the sort: those at or above a threshold refined
+  // inside the threshold's log2 bin (key <= kfine), nfine of them
+  rkey kfine;
+  uint64_t nfine;
 };
 #define POP_RHO_LESS 1u         // fewer valid cells than ncell
 #define POP_NO_BIN 2u           // the histogram never reaches ncell
@@ -147,6 +152,13 @@ hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
 // from the bias histogram and the valid-cell count *nvalid.
 hipError_t launch_select(const unsigned long long *hist, const uint64_t *nvalid, PopCtl *ctl,
     hipStream_t st);
+// Refinement of the candidates (keys[0, ctl->nsort)) inside the threshold
+// bin by the top four mantissa bits: 16 counts (sub, cleared by the caller),
+// the refined threshold and count (ctl->kfine, ctl->nfine), and the kept
+// keys into out (counter: one u64, cleared by the caller); kcap: the key
+// capacity, which sizes the grids.
+hipError_t launch_refine(const rkey *keys, size_t kcap, PopCtl *ctl, unsigned long long *sub,
+    rkey *out, unsigned long long *counter, hipStream_t st);
 hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
     uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl = nullptr);
 hipError_t launch_compact_at(const real *rhot, real thres, size_t N,

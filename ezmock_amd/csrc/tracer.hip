@@ -317,6 +317,8 @@ k_select(const unsigned long long *__restrict__ hist, const uint64_t *__restrict
   ctl->nvalid = nvalid;
   ctl->cnt = c;
   ctl->nsort = c > ctl->kcap ? ctl->kcap : c;
+  ctl->nfine = ctl->nsort;
+  ctl->kfine = ~(rkey) 0;
   if (c > ctl->kcap) fl |= POP_KEYS_FULL;
   ctl->flags = fl;
 }
@@ -482,6 +484,137 @@ hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
   return hipGetLastError();
 }
 
+// ------------------------------------------------- candidate refinement
+// The threshold bin imax covers [t, 2t), t = rho_thres a power of two
+// (except bin 0, open above, and the last bin, REAL_MIN: not refined).  Its
+// candidates are counted by their top four mantissa bits; the refined
+// threshold t (1 + s/16) is the largest whose count of candidates at or
+// above it still reaches ncell, so the ncell-th largest density and its
+// ties stay above it and the walk over the kept keys gives the same PDF.
+#ifdef SINGLE_PREC
+#define KEY_MANT 23
+#else
+#define KEY_MANT 52
+#endif
+
+__device__ __forceinline__ bool refine_on(const PopCtl *ctl) {
+  return ctl->imax > 0 && ctl->imax < EZ_NUM_LOGB_BIN - 1 && !(ctl->flags & POP_KEYS_FULL);
+}
+
+__global__ void __launch_bounds__(TPB)
+k_key_subhist(const rkey *__restrict__ keys, const PopCtl *__restrict__ ctl,
+    unsigned long long *__restrict__ sub) {
+  __shared__ uint32_t h[16];
+  if (!refine_on(ctl)) return;                      // uniform
+  const uint64_t n = ctl->nsort;
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  if (b0 >= n) return;
+  if (threadIdx.x < 16) h[threadIdx.x] = 0;
+  __syncthreads();
+  const rkey tb = ~real_key(ctl->rho_thres);        // bits of t
+  const rkey eb = tb >> KEY_MANT;                   // its exponent field
+  rkey kv[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    kv[q] = keys[i < n ? i : n - 1];
+  }
+  // counts in registers, summed over the wave, one LDS add per bin and wave
+  // (the keys crowd into few sub-bins: per-key atomics serialise)
+  uint32_t c[16];
+#pragma unroll
+  for (int b = 0; b < 16; b++) c[b] = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) q * TPB + threadIdx.x;
+    const rkey vb = ~kv[q];
+    const bool in = i < n && (vb >> KEY_MANT) == eb;
+    const uint32_t sb = (uint32_t) (vb >> (KEY_MANT - 4)) & 15u;
+#pragma unroll
+    for (int b = 0; b < 16; b++) c[b] += (in && sb == (uint32_t) b) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int b = 0; b < 16; b++) {
+    uint32_t v = c[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&h[b], v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&sub[threadIdx.x], (unsigned long long) h[threadIdx.x]);
+}
+
+__device__ __forceinline__ void refine_threshold(const unsigned long long *__restrict__ sub,
+    PopCtl *__restrict__ ctl) {
+  uint64_t inbin = 0;
+  for (int s = 0; s < 16; s++) inbin += sub[s];
+  uint64_t acc = ctl->nsort - inbin;                // candidates above the bin
+  int s = 15;
+  for (; s > 0; s--) {
+    if (acc + sub[s] >= ctl->ncell) break;
+    acc += sub[s];
+  }
+  acc += sub[s];
+  const rkey tb = ~real_key(ctl->rho_thres);
+  ctl->kfine = ~(tb | ((rkey) s << (KEY_MANT - 4)));
+  ctl->nfine = acc;
+}
+
+__global__ void __launch_bounds__(64)
+k_refine_select(const unsigned long long *__restrict__ sub, PopCtl *__restrict__ ctl) {
+  if (threadIdx.x) return;
+  if (refine_on(ctl)) refine_threshold(sub, ctl);
+  if (ctl->nfine > ctl->kfcap) ctl->flags |= POP_KEYS_FULL;
+}
+
+
+// Keys at or above the refined threshold (key <= kfine), in any order (the
+// sort that follows orders them): one device atomic per block reserves the
+// block's range.
+__global__ void __launch_bounds__(TPB)
+k_key_keep(const rkey *__restrict__ keys, const PopCtl *__restrict__ ctl,
+    rkey *__restrict__ out, unsigned long long *__restrict__ counter) {
+  __shared__ uint32_t ws[TPB / 64];
+  __shared__ unsigned long long base;
+  const uint64_t n = ctl->nsort;
+  const size_t b0 = (size_t) blockIdx.x * CPB;
+  if (b0 >= n) return;
+  const rkey kf = ctl->kfine;
+  rkey kv[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) threadIdx.x * CPT + q;
+    kv[q] = keys[i < n ? i : n - 1];
+  }
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) threadIdx.x * CPT + q;
+    if (i < n && kv[q] <= kf) c++;
+  }
+  uint32_t tot;
+  uint32_t pos = block_excl_scan<uint32_t>(c, tot, ws);
+  if (threadIdx.x == 0) base = tot ? atomicAdd(counter, (unsigned long long) tot) : 0;
+  __syncthreads();
+  const unsigned long long bb = base;
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const size_t i = b0 + (size_t) threadIdx.x * CPT + q;
+    if (i < n && kv[q] <= kf) out[bb + pos++] = kv[q];
+  }
+}
+
+hipError_t launch_refine(const rkey *keys, size_t kcap, PopCtl *ctl, unsigned long long *sub,
+    rkey *out, unsigned long long *counter, hipStream_t st) {
+  const unsigned nb = (unsigned) ez_blocks(kcap, CPB);
+  hipLaunchKernelGGL(k_key_subhist, dim3(nb), dim3(TPB), 0, st, keys, (const PopCtl *) ctl, sub);
+  hipLaunchKernelGGL(k_refine_select, dim3(1), dim3(64), 0, st, (const unsigned long long *) sub,
+      ctl);
+  hipLaunchKernelGGL(k_key_keep, dim3(nb), dim3(TPB), 0, st, keys, (const PopCtl *) ctl, out,
+      counter);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------- radix sort
 // LSD, 8-bit digits, 4096 keys per block; stable ranks within a block from
 // wave ballots (lanes with equal digit) and per-wave digit counts.
@@ -608,7 +741,7 @@ k_walk(const rkey *__restrict__ sorted, size_t M, const uint64_t *__restrict__ n
     const uint64_t *__restrict__ ncand) {
   const int lane = threadIdx.x;
   if (ctl) {
-    M = ctl->nsort;
+    M = ctl->nfine;
     nmax = ctl->nmax;
   }
   size_t s = 0;
@@ -889,7 +1022,11 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     sigma = ctl->sigma_v;
   }
   const size_t npair = (ntot + 1) / 2;
-  const size_t q0blk = (size_t) xcd_item(blockIdx.x, gridDim.x) * TPB * PPT;
+  // the XCD-aware order over the blocks the catalogue needs (the grid may
+  // be sized by a capacity), so that every XCD gets its share
+  const unsigned nact = (unsigned) ((npair + (size_t) TPB * PPT - 1) / ((size_t) TPB * PPT));
+  if (blockIdx.x >= nact) return;       // uniform: blocks past the catalogue
+  const size_t q0blk = (size_t) xcd_item(blockIdx.x, nact) * TPB * PPT;
   if (threadIdx.x == 0 && !mtw) {
     MrgState g = seed;
     mrg_jump(g, base_draw + 6 * q0blk, jtab);
