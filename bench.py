@@ -113,8 +113,8 @@ def run_large(Ng=1536, L=3000.0):
     dt = time.perf_counter() - t0
     kt = ez.kernel_times()
     gbs = {}
-    for name in ("fft_pass_i", "fft_pass_j", "fft_rows_c2r", "cic_bin_count", "cic_bin_scatter",
-                 "cic_tiles_rho_mom", "bias"):
+    for name in ("fft_pass_i", "fft_pass_i_gen", "fft_pass_j", "fft_rows_c2r", "cic_bin_count",
+                 "cic_bin_scatter", "cic_tiles_rho_mom", "bias"):
         if name in kt:
             t, c = kt[name]
             b = algorithmic_bytes(name, Ng, 4, ntr)
