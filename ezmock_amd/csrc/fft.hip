@@ -410,19 +410,24 @@ k_fft_first3_gen(C *__restrict__ out0, C *__restrict__ out1, C *__restrict__ out
   const size_t base = (size_t) o * Ngh + kk0;
   const int ko = (o <= Ngh) ? o : o - Ng;
   {
+    // work units: PR parts of TK / PR cells per row, so that every thread
+    // draws when the workgroup has more threads than rows
+    constexpr int PR = (NT >= 2 * L && TK >= 2) ? ((NT >= 4 * L && TK >= 4) ? 4 : 2) : 1;
+    constexpr int CU = TK / PR;
     const MrgState seed = seedp ? *seedp : gp.seed_state;
     const uint64_t mult = gp.fixamp ? 1 : 2;
     const size_t Ngk = (size_t) Ngh + 1;
-    const int c0 = (kk0 == 0) ? 1 : 0;      // kz = 0: from slot0
 #pragma unroll 1
-    for (int n = threadIdx.x; n < L; n += NT) {
+    for (int w = threadIdx.x; w < L * PR; w += NT) {
+      const int n = w / PR, ca = (w % PR) * CU;
       C *row = lds + n * TK;
       const int sw = n & (TK - 1);
+      const int c0 = (kk0 == 0 && ca == 0) ? 1 : 0;   // kz = 0: from slot0
       if (c0) row[sw] = C{0, 0};
       MrgState g = seed;
-      mrg_jump(g, (((size_t) n * Ng + o) * Ngk + kk0 + c0 - 1) * mult, jtab);
+      mrg_jump(g, (((size_t) n * Ng + o) * Ngk + kk0 + ca + c0 - 1) * mult, jtab);
 #pragma unroll 1
-      for (int c = c0; c < TK; c++) {
+      for (int c = ca + c0; c < ca + CU; c++) {
         const double amp = gp.fixamp ? 1.0 : mrg_next(g);
         const double ph = mrg_next(g);
         const GenCell gc = gen_cell(n, o, kk0 + c, amp, ph, gp, ptab);
@@ -697,7 +702,7 @@ static hipError_t launch_first3_gen_tk(cplx *o0, cplx *o1, cplx *o2, const cplx 
     const cplx *tw, int Ng, int Ngh, const GenParams &gp, const double *ptab,
     const uint32_t *jtab, const MrgState *seedp, hipStream_t st) {
   constexpr int E = L * T;
-  constexpr int NT = pass_threads<E>();
+  constexpr int NT = pass_threads<E>();    // (4 per thread: 1.33 against 1.04 ms at 512^3)
   hipLaunchKernelGGL((k_fft_first3_gen<cplx, L, T, NT>),
       dim3((unsigned) ((Ngh / T) * (size_t) Ng)), dim3(NT), 0, st, o0, o1, o2, slot0, tw,
       Ng, Ngh, gp, ptab, jtab, seedp);
