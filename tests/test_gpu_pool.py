@@ -32,13 +32,17 @@ def _alone(pk, Ng, L, seed, params, nexp, rsd, dk, nbin):
     return cat, pkr
 
 
-def test_pool_result_same_alone_and_in_mix(gpu_available, pk_table):
+@pytest.mark.parametrize("graphs", [False, True])
+def test_pool_result_same_alone_and_in_mix(gpu_available, pk_table, graphs):
+    """Plain launches (the pool's default) and every instance replaying its
+    captured HIP graphs (pool option 8 = 1)."""
     k, P = pk_table
     target = dict(Ngrid=256, Lbox=1000.0, seed=7, params=(0.5, 10.0, 0.3, 150.0),
                   nexp=300000, rsd_fac=0.011384, pk_dk=0.01, pk_nbin=40)
     cat0, pk0 = _alone(pk_table, 256, 1000.0, 7, target["params"], 300000,
                        0.011384, 0.01, 40)
-    pool = MockPool(k, P, Z, OM, nslot=4, mem_budget=100e9)
+    pool = MockPool(k, P, Z, OM, nslot=4, mem_budget=100e9,
+                    options={8: 1} if graphs else None)
     rs = np.random.default_rng(3)
     ids = []
     for i in range(12):
