@@ -160,6 +160,33 @@ def test_populate_validation_before_gpu(pk_table):
     assert str(e.value) == "EZmock power spectrum uninitialized"
 
 
+def test_pool_admission_checks_before_gpu():
+    """The request pool's checks that run before any device work: slot count,
+    seed 0 and requests over the memory budget are refused at submit; options
+    are accepted only while nothing has been admitted (every instance is
+    created with the same options)."""
+    from ezmock_amd.pool import MockPool
+    L = _lib.load("f32")
+    k = np.logspace(-3, 1, 64)
+    with pytest.raises(RuntimeError) as e:
+        MockPool(k, k ** -1.5, 0.5, 0.31, nslot=0)
+    assert str(e.value) == L.EZmock_errmsg(11).decode()          # ARG_THREAD
+    assert L.EZmock_ext_pool_option(None, 8, 1) == 12             # ARG_NULL
+    pool = MockPool(k, k ** -1.5, 0.5, 0.31, nslot=2, mem_budget=1e9, options={8: 1, 9: 0})
+    try:
+        with pytest.raises(RuntimeError) as e:
+            pool.submit(64, 500., 0, (0.5, 10., 0.3, 150.), 5000)
+        assert str(e.value) == L.EZmock_errmsg(10).decode()      # ARG_SEED
+        assert pool.footprint(2048) > 1e9
+        with pytest.raises(RuntimeError) as e:
+            pool.submit(2048, 2500., 1, (0.5, 10., 0.3, 150.), 5000)
+        assert str(e.value) == L.EZmock_errmsg(1).decode()       # MEMORY
+        # refused requests admit nothing: options are still open
+        assert L.EZmock_ext_pool_option(pool._p, 7, 1) == 0
+    finally:
+        pool.close()
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("rsd", [0.0, 0.011384])
 def test_write_ascii_formats(tmp_path, prec, rsd):
