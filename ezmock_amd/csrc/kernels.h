@@ -58,6 +58,17 @@ struct CicGeom {
   int nbx, nby, nbz;
 };
 CicGeom cic_geometry(int Ng, real igs);
+// Slot of cell (x, y, z) in the velocity gather's field {rho_c, u0, u1, u2}:
+// 2 x 2 x 2 bricks of consecutive slots (one 128-byte line in FP32), bricks
+// in cell-major order of (x/2, y/2, z/2).  A tracer's eight interpolation
+// corners span 1.5^3 = 3.4 lines on average instead of the 4.5 lines of
+// the cell-major layout (two z-neighbours per line pair, 1/8 of them split).
+// Every supported grid size is even.
+__host__ __device__ __forceinline__ size_t vel_brick_index(int x, int y, int z, int Ng) {
+  const size_t h = (size_t) (Ng >> 1);
+  return ((((size_t) (x >> 1) * h + (size_t) (y >> 1)) * h + (size_t) (z >> 1)) << 3)
+         | (size_t) (((x & 1) << 2) | ((y & 1) << 1) | (z & 1));
+}
 size_t cic_nkeys(const CicGeom &g);
 size_t cic_nblocks(const CicGeom &g);
 // Words of the scan key array: displacement bounds (3 minima, 3 maxima as
@@ -89,6 +100,7 @@ hipError_t launch_cic_fixed_init(uint64_t *cursor, const CicGeom &g, uint32_t ca
 // vel (optional): per cell {rho_c, u0, u1, u2}, u_a = m_a / rho_c where
 // rho_c > REAL_TOL and m_a otherwise (m_a the painted momentum); rho_c is
 // the density painted in the same pass, or rho_norm when rho is null.
+// vel is stored in 2 x 2 x 2 bricks (vel_brick_index).
 // off: exact ranges [off[t], off[t + 1]) (cap = 0), or the cursor array of
 // fixed ranges [t cap, min(cursor[t], (t + 1) cap)) (cap > 0).
 hipError_t launch_cic_tiles(const real *p0, const real *p1, const real *p2,

@@ -1071,10 +1071,10 @@ __device__ __forceinline__ void corners_load(const real4 *__restrict__ vel,
   const int y1 = (y0 == Ng - 1) ? 0 : y0 + 1;
   const int z1 = (z0 == Ng - 1) ? 0 : z0 + 1;
   const size_t c[8] = {
-    ((size_t) x0 * Ng + y0) * Ng + z0, ((size_t) x0 * Ng + y0) * Ng + z1,
-    ((size_t) x0 * Ng + y1) * Ng + z0, ((size_t) x0 * Ng + y1) * Ng + z1,
-    ((size_t) x1 * Ng + y0) * Ng + z0, ((size_t) x1 * Ng + y0) * Ng + z1,
-    ((size_t) x1 * Ng + y1) * Ng + z0, ((size_t) x1 * Ng + y1) * Ng + z1};
+    vel_brick_index(x0, y0, z0, Ng), vel_brick_index(x0, y0, z1, Ng),
+    vel_brick_index(x0, y1, z0, Ng), vel_brick_index(x0, y1, z1, Ng),
+    vel_brick_index(x1, y0, z0, Ng), vel_brick_index(x1, y0, z1, Ng),
+    vel_brick_index(x1, y1, z0, Ng), vel_brick_index(x1, y1, z1, Ng)};
   // per corner {rho, u0, u1, u2}: u already normalised by rho (disp_cic)
 #pragma unroll
   for (int q = 0; q < 8; q++) k.f[q] = vel[c[q]];
