@@ -145,7 +145,6 @@ struct Mesh {
   // staging (parameters in, results out), spectrum seed of a replayed mock
   PopCtl *ctl_d = nullptr;
   MrgState *seed_d = nullptr;
-  unsigned long long *sub16 = nullptr;  // candidate counts by sub-bin
   struct PopIO {
     PopCtl in;
     MrgState seed;
@@ -268,7 +267,7 @@ void drop_graphs(Mesh *m) {
 void free_mesh_device(Mesh *m) {
   if (m->smt) (void) hipStreamSynchronize(m->smt);   // window builds ahead of use
   drop_graphs(m);
-  dfree(&m->ctl_d); dfree(&m->seed_d); dfree(&m->sub16);
+  dfree(&m->ctl_d); dfree(&m->seed_d);
   if (m->pio) { (void) hipHostFree(m->pio); m->pio = nullptr; }
   for (int i = 0; i < 3; i++) dfree(&m->psi[i]);
   dfree(&m->vel4); dfree(&m->gen_base);
@@ -965,7 +964,7 @@ size_t dsel_kfcap(const Mesh *m) { return m->keys_cap / 3 ? m->keys_cap / 3 : 1;
 int dsel_upload(Mesh *m, const BiasParams &bp, const std::vector<size_t> &num, size_t ncell,
     real sigma_v) {
   int e;
-  if ((e = dalloc(&m->ctl_d, 1)) || (e = dalloc(&m->sub16, 16))) return e;
+  if ((e = dalloc(&m->ctl_d, 1))) return e;
   if (!m->pio && hipHostMalloc((void **) &m->pio, sizeof(Mesh::PopIO)) != hipSuccess) {
     m->pio = nullptr;
     return EZMOCK_ERR_MEMORY;
@@ -1037,11 +1036,10 @@ int dsel_launch(EZMOCK *ez, Mesh *m, bool vel) {
   }
   // the candidates at or above a threshold refined inside the threshold
   // bin, into keys_tmp (about a third of them at 512^3), then sorted there
-  KCHK(launch_zero(m->sub16, 16 * sizeof(unsigned long long), m->st));
+  // (the refinement's block counts and offsets in the sort's buffers)
   {
     Scope s(m, "refine");
-    KCHK(launch_refine(m->keys, m->keys_cap, ctl, m->sub16, m->keys_tmp, m->counter + 1,
-        m->st));
+    KCHK(launch_refine(m->keys, m->keys_cap, ctl, m->sort_gh, m->sort_go, m->keys_tmp, m->st));
   }
   {
     Scope s(m, "sort");

@@ -165,12 +165,13 @@ hipError_t launch_bias(const real *rho, real *rhot, const uint64_t *blockoff,
 hipError_t launch_select(const unsigned long long *hist, const uint64_t *nvalid, PopCtl *ctl,
     hipStream_t st);
 // Refinement of the candidates (keys[0, ctl->nsort)) inside the threshold
-// bin by the top four mantissa bits: 16 counts (sub, cleared by the caller),
-// the refined threshold and count (ctl->kfine, ctl->nfine), and the kept
-// keys into out (counter: one u64, cleared by the caller); kcap: the key
-// capacity, which sizes the grids.
-hipError_t launch_refine(const rkey *keys, size_t kcap, PopCtl *ctl, unsigned long long *sub,
-    rkey *out, unsigned long long *counter, hipStream_t st);
+// bin by the top four mantissa bits: the refined threshold and count
+// (ctl->kfine, ctl->nfine), and the kept keys into out in key-block order;
+// kcap: the key capacity, which sizes the grids; bc (17 u32 per block of
+// CPB keys) and boff (one u64 per block) are scratch, written before read
+// (the sort's digit counts and offsets serve).
+hipError_t launch_refine(const rkey *keys, size_t kcap, PopCtl *ctl, uint32_t *bc,
+    uint64_t *boff, rkey *out, hipStream_t st);
 hipError_t launch_block_cand(const uint32_t *blockhist, size_t nblk, int imax,
     uint32_t *blockcnt, hipStream_t st, const PopCtl *ctl = nullptr);
 hipError_t launch_compact_at(const real *rhot, real thres, size_t N,
