@@ -1184,7 +1184,6 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
     const uint32_t *__restrict__ jtab, const uint32_t *__restrict__ mtw,
     real *__restrict__ vx,
     real *__restrict__ vy, real *__restrict__ vz, const PopCtl *__restrict__ ctl) {
-  __shared__ MrgState base;
   if (ctl) {                            // ntot: the catalogue capacity
     if (ctl->ntot < ntot) ntot = ctl->ntot;
     seed = ctl->vel_seed;
@@ -1196,20 +1195,14 @@ k_velocities(const real *__restrict__ x, const real *__restrict__ y,
   const unsigned nact = (unsigned) ((npair + (size_t) TPB * PPT - 1) / ((size_t) TPB * PPT));
   if (blockIdx.x >= nact) return;       // uniform: blocks past the catalogue
   const size_t q0blk = (size_t) xcd_item(blockIdx.x, nact) * TPB * PPT;
-  if (threadIdx.x == 0 && !mtw) {
-    MrgState g = seed;
-    mrg_jump(g, base_draw + 6 * q0blk, jtab);
-    base = g;
-  }
-  __syncthreads();
   const size_t q0 = q0blk + (size_t) threadIdx.x * PPT;
   if (q0 >= npair) return;
   Draws g;
   g.w = mtw;
   g.d = 6 * q0;                         // MT19937 words start at draw 8N
-  if (!mtw) {
-    g.g = base;
-    mrg_jump(g.g, 6 * (q0 - q0blk), jtab);
+  if (!mtw) {                           // one jump per thread from the stream
+    g.g = seed;
+    mrg_jump(g.g, base_draw + 6 * q0, jtab);
   }
 #pragma unroll 1
   for (int p = 0; p < PPT; p++) {
